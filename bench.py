@@ -1,0 +1,268 @@
+#!/usr/bin/env python3
+"""FDK benchmark on one MI355X: backprojection GUPS and FDK seconds, FP16-stored
+vs FP32 filtered projections, for the 1024^3 workload (1800 x 2048^2).
+
+One "step" = one pass of the hot path over the resident projection stack:
+cosine weight + ramp filter (one LDS FFT kernel), repack into the storage type,
+backprojection of the whole volume (fdk_calculate_reconstruction without a slab
+sink: the input is already in HBM and the volume stays there).  Stage times come
+from HIP events recorded on the engine's stream around each launch.
+
+Prints ONE JSON line.  --gpus N with N > 1 exits: this build runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from tomokvas_amd import _lib, geometry  # noqa: E402
+from tomokvas_amd.engine import FilteredBackProjection  # noqa: E402
+
+METRIC = ("backprojection GUPS + FDK seconds @1024³: FP16-store vs FP32, "
+          "rel-RMSE≤1e-3")
+L2_PEAK_GBS = 8 * 16 * 128 * 2.4  # 8 XCD x 16 channels x 128 B/clk x 2.4 GHz = 39321.6 GB/s
+HBM_PEAK_GBS = 8000.0
+NAMES = {_lib.F32: "f32", _lib.F16: "f16", _lib.BF16: "bf16"}
+CELL_BYTES = {_lib.F32: 4, _lib.F16: 2, _lib.BF16: 2}   # bytes per stored sample
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def hip():
+    _lib.load()
+    h = C.CDLL("libamdhip64.so.7")
+    h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    h.hipFree.argtypes = [C.c_void_p]
+    h.hipDeviceSynchronize.argtypes = []
+    return h
+
+
+def read_traffic(workload: str, storage: str):
+    """HBM bytes per backprojection launch measured by rocprofv3 --pmc
+    (profiles/bp_traffic_<workload>.txt, lines 'storage fetch_bytes write_bytes')."""
+    path = os.path.join(ROOT, "profiles", f"bp_traffic_{workload}.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.split()
+        if len(parts) == 3 and parts[0] == storage and not line.startswith("#"):
+            return float(parts[1]) + float(parts[2])
+    return None
+
+
+def sub_cube(g, seed: int, n: int = 64):
+    """Seeded n^3 voxel block whose centre lies within 256 px of the isocentre."""
+    rng = np.random.default_rng(seed)
+    half = int(256 / g.scale)
+    cx, cy, cz = g.Nx // 2, g.Ny // 2, g.Nz // 2
+    x0 = int(np.clip(cx + rng.integers(-half, half) - n // 2, 0, g.Nx - n))
+    y0 = int(np.clip(cy + rng.integers(-half, half) - n // 2, 0, g.Ny - n))
+    z0 = int(np.clip(cz + rng.integers(-half, half) - n // 2, 0, g.Nz - n))
+    return x0, y0, z0, n
+
+
+def detector_rows_of_block(g, x0, y0, z0, n):
+    """Row range [v_lo, v_hi) containing every sample the block reads."""
+    ax = g.scale * (np.arange(x0, x0 + n) - g.cx)
+    ay = g.scale * (np.arange(y0, y0 + n) - g.cy)
+    az = g.scale * (np.array([z0, z0 + n - 1]) - (g.Nz - 1) / 2 - g.cz)
+    b = np.arange(g.Nang) * g.dbeta + g.rot
+    X, Y = np.meshgrid(ax, ay)
+    S = X.ravel()[:, None] * np.cos(b)[None, :] + Y.ravel()[:, None] * np.sin(b)[None, :]
+    U = g.D / (g.D - S)
+    v = np.concatenate([(az[0] * U).ravel(), (az[1] * U).ravel()]) + g.cZ
+    lo = max(0, int(math.floor(v.min())) - 2)
+    hi = min(g.Nv, int(math.ceil(v.max())) + 3)
+    return lo, hi
+
+
+def cpu_leg(fbp, w, g, block_vols, hipm, threads):
+    """CPU baseline (FP32 mirror, timed) and FP64 oracle on a seeded 64^3 block."""
+    import oracle as O
+    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    x0, y0, z0, n = sub_cube(g, w.seed)
+    lo, hi = detector_rows_of_block(g, x0, y0, z0, n)
+    rows = hi - lo
+    log(f"cpu leg: block x{x0} y{y0} z{z0} n{n}, detector rows [{lo},{hi})")
+    raw = np.empty((rows, g.Nang, g.Nu), np.float32)
+    src = fbp.device_sinogram_ptr() + lo * g.Nang * g.Nu * 4
+    rc = hipm.hipMemcpy(raw.ctypes.data, C.c_void_p(src), raw.nbytes, 2)
+    assert rc == 0, "hipMemcpy D2H failed"
+    gd = g.oracle_dict()
+    gd.update(Nv=rows, cZ=g.cZ - lo)
+    og = O.Geometry(**gd)
+    zz, yy, xx = np.meshgrid(np.arange(z0, z0 + n), np.arange(y0, y0 + n),
+                             np.arange(x0, x0 + n), indexing="ij")
+    pts = np.stack([xx.ravel(), yy.ravel(), zz.ravel()], axis=1).astype(np.int32)
+    Hf = O.filter_spectrum(w.filter, g.Nu, g.tau)
+    # FP32 mirror: the timed CPU baseline (weight + filter of the rows + BP of the block)
+    t0 = time.perf_counter()
+    f32 = O.filter_rows(og, O.weight(og, raw), Hf)
+    t1 = time.perf_counter()
+    m32 = O.backproject_points(og, f32, pts)
+    t2 = time.perf_counter()
+    # FP64 oracle on the same raw projections
+    f64 = O.filter_rows(og, O.weight(og, raw.astype(np.float64), f64=True), Hf, f64=True)
+    o64 = O.backproject_points(og, f64, pts, f64=True)
+    t3 = time.perf_counter()
+    del f32, f64, raw
+    upd = pts.shape[0] * g.Nang
+    out = {
+        "value": upd / (t2 - t1) / 1e9, "unit": "GUPS", "cores": threads, "kind": "port",
+        "sample": (f"FP32 OpenMP mirror, backprojection of a seeded {n}^3 voxel block "
+                   f"(x{x0} y{y0} z{z0}) over all {g.Nang} projections = {upd:.3e} voxel-updates; "
+                   f"weight+filter of its {rows} detector rows x {g.Nang} projections took "
+                   f"{t1 - t0:.2f} s"),
+        "bp_seconds": t2 - t1, "filter_seconds": t1 - t0, "fp64_oracle_seconds": t3 - t2,
+    }
+    errs = {"mirror_f32_vs_fp64": O.rel_rmse(m32, o64)}
+    for st, vol in block_vols.items():
+        sub = vol[:, y0:y0 + n, x0:x0 + n].reshape(-1)
+        errs[f"gpu_{st}_vs_fp64_oracle_block"] = O.rel_rmse(sub, o64)
+        if st == "f32":
+            errs["gpu_f32_vs_mirror_block"] = O.rel_rmse(sub, m32)
+    return out, errs, (x0, y0, z0, n)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="cfg3")
+    ap.add_argument("--storages", default="f32,f16,bf16")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the FDK path "
+              "is not sharded across GPUs in this build", file=sys.stderr)
+        sys.exit(2)
+
+    w = geometry.workloads()[args.workload]
+    g = w.geom
+    hipm = hip()
+    fbp = FilteredBackProjection(0)
+    g.apply(fbp, None, None)
+    _lib.check(fbp.SetFilter(w.filter), "SetFilter")
+    _lib.check(fbp.SetStorage(_lib.F32), "SetStorage")
+    t0 = time.perf_counter()
+    _lib.check(fbp.Initialize(), "Initialize")
+    log(f"initialised {w.name}: {time.perf_counter() - t0:.2f} s")
+
+    L = _lib.load()
+    ells = np.ascontiguousarray(w.ellipsoids())
+    t0 = time.perf_counter()
+    _lib.check(L.fdk_phantom_project(fbp.handle, ells.ctypes.data, ells.shape[0], 0, g.Nang,
+                                     fbp.device_sinogram_ptr(), None), "phantom")
+    if w.I0:
+        _lib.check(L.fdk_poisson_log(fbp.device_sinogram_ptr(), g.Nu * g.Nv * g.Nang, w.I0,
+                                     w.seed, None), "poisson")
+    hipm.hipDeviceSynchronize()
+    log(f"synthetic projections on the GPU: {time.perf_counter() - t0:.2f} s")
+
+    nvol = g.Nx * g.Ny * g.Nz
+    ref = C.c_void_p()
+    assert hipm.hipMalloc(C.byref(ref), nvol * 4) == 0
+    nullcb = _lib.GET_BLOCK_FN()
+    order = [{"f32": _lib.F32, "f16": _lib.F16, "bf16": _lib.BF16}[s]
+             for s in args.storages.split(",")]
+    if _lib.F32 in order:
+        order.remove(_lib.F32)
+        order.insert(0, _lib.F32)
+    res = {}
+    block = sub_cube(g, w.seed)
+    block_vols = {}
+    for st in order:
+        _lib.check(fbp.SetStorage(st), "SetStorage")
+        for _ in range(args.warmup):
+            _lib.check(L.fdk_calculate_reconstruction(fbp.handle, nullcb, None), "warmup")
+        hipm.hipDeviceSynchronize()
+        walls, stages = [], []
+        for _ in range(args.steps):
+            t0 = time.perf_counter()
+            _lib.check(L.fdk_calculate_reconstruction(fbp.handle, nullcb, None), "step")
+            hipm.hipDeviceSynchronize()
+            walls.append(time.perf_counter() - t0)
+            stages.append(fbp.stage_times_ms())
+        s = np.array(stages)
+        filt_ms, pack_ms, bp_ms = (float(np.mean(s[:, i])) for i in range(3))
+        fdk_ms = filt_ms + pack_ms + bp_ms
+        dvol = fbp.device_volume_ptr()
+        cmp = None
+        if st == _lib.F32:
+            assert hipm.hipMemcpy(ref, C.c_void_p(dvol), nvol * 4, 3) == 0
+        else:
+            out3 = (C.c_double * 3)()
+            _lib.check(L.fdk_compare(C.c_void_p(dvol), ref, nvol, out3, None), "compare")
+            cmp = math.sqrt(out3[0] / out3[1])
+        x0, y0, z0, n = block
+        slab = np.empty((n, g.Ny, g.Nx), np.float32)
+        assert hipm.hipMemcpy(slab.ctypes.data, C.c_void_p(dvol + z0 * g.Nx * g.Ny * 4),
+                              slab.nbytes, 2) == 0
+        block_vols[NAMES[st]] = slab
+        res[NAMES[st]] = dict(filter_ms=filt_ms, pack_ms=pack_ms, bp_ms=bp_ms, fdk_ms=fdk_ms,
+                              wall_ms=1e3 * float(np.mean(walls)),
+                              gups=g.voxel_updates / (bp_ms * 1e-3) / 1e9,
+                              rel_rmse_vs_f32=cmp)
+        log(f"{NAMES[st]}: filter {filt_ms:.2f} ms, pack {pack_ms:.2f} ms, BP {bp_ms:.2f} ms "
+            f"({res[NAMES[st]]['gups']:.1f} GUPS), FDK {fdk_ms:.2f} ms, vs f32 {cmp}")
+    hipm.hipFree(ref)
+
+    head = "f16" if "f16" in res else NAMES[order[0]]
+    r = res[head]
+    upd = g.voxel_updates
+    s_bytes = CELL_BYTES[{"f32": _lib.F32, "f16": _lib.F16, "bf16": _lib.BF16}[head]]
+    gather = 4 * s_bytes * upd                      # 2x2 taps per voxel-update
+    achieved = gather / (r["bp_ms"] * 1e-3) / 1e9   # GB/s
+    traffic = read_traffic(args.workload, head)
+    roofline = {
+        "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+        "kernel": "bp_kernel", "algorithmic_bytes": gather,
+        "l2_peak": L2_PEAK_GBS, "frac_l2": achieved / L2_PEAK_GBS,
+        "compulsory_hbm_bytes": g.Nang * g.Nv * g.Nu * s_bytes + 4 * g.Nx * g.Ny * g.Nz,
+    }
+    out = {
+        "metric": METRIC, "value": r["gups"], "unit": "GUPS", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": r["fdk_ms"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": f"{head}-store/f32-accumulate",
+        "data": "synthetic: 64 seeded random ellipsoids, exact line integrals + Poisson(I0=1e5) "
+                "counts, generated on the GPU",
+        "config": w.describe(),
+        "fdk_seconds": {k: v["fdk_ms"] / 1e3 for k, v in res.items()},
+        "bp_gups": {k: v["gups"] for k, v in res.items()},
+        "stage_ms": res,
+        "speedup_fdk_f16_vs_f32": (res["f32"]["fdk_ms"] / res["f16"]["fdk_ms"]
+                                   if "f16" in res and "f32" in res else None),
+        "rel_rmse": {f"{k}_vs_f32_full_volume": v["rel_rmse_vs_f32"] for k, v in res.items()
+                     if v["rel_rmse_vs_f32"] is not None},
+        "roofline": roofline,
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        t0 = time.perf_counter()
+        cpu, errs, blk = cpu_leg(fbp, w, g, block_vols, hipm, args.threads)
+        out["cpu_baseline"] = cpu
+        out["rel_rmse"].update(errs)
+        log(f"cpu leg {time.perf_counter() - t0:.1f} s: {cpu['value']:.3f} GUPS, errors {errs}")
+    fbp.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,313 @@
+"""GPU parity of the HIP FDK path against the CPU oracle (through the C ABI).
+
+Bars: FP32 path vs the FP32 mirror <= 1e-5 relative RMSE;
+stage-level FP32 kernels much tighter; FP64 instantiations vs the FP64 oracle
+<= 1e-12; FP16/BF16 storage <= 1e-3 vs the FP64 oracle.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from tomokvas_amd import _lib, geometry, phantoms
+from tomokvas_amd.engine import VolumeSink
+
+from gpu_util import engine, ogeom, run_engine, say, scan, stages, ulp_diff
+
+pytestmark = pytest.mark.gpu
+
+TOL_FDK = 1e-5      # FP32 GPU vs FP32 CPU mirror (relative RMSE)
+TOL_F16 = 1e-3      # FP16/BF16 storage vs FP64 oracle (relative RMSE)
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as t
+    assert t.cuda.is_available(), "GPU tests need a HIP device"
+    return t
+
+
+@pytest.fixture(scope="module")
+def cfg1():
+    w = geometry.workloads()["cfg1"]
+    g = w.geom
+    sino = O.project_ellipsoids(ogeom(g), O.shepp_logan(g.fov_px / 2))
+    return g, sino
+
+
+# ------------------------------------------------------------------ engine, cfg1
+def test_engine_fdk_cfg1_matches_mirror(cfg1):
+    g, sino = cfg1
+    say("engine FDK cfg1 (128^3, 180 x 128^2)")
+    vol, times, sink = run_engine(g, sino.astype(np.float32))
+    ref32 = O.fdk(ogeom(g), sino.astype(np.float32), kind=0)
+    ref64 = O.fdk(ogeom(g), sino, kind=0, f64=True)
+    e32 = O.rel_rmse(vol, ref32)
+    e64 = O.rel_rmse(vol, ref64)
+    say(f"  rel RMSE vs FP32 mirror {e32:.3e}, vs FP64 oracle {e64:.3e}, stage ms {times}")
+    assert e32 <= TOL_FDK
+    assert e64 <= TOL_FDK
+    assert sink.calls == [(n, False) for n in range(O.slab_count(g.Nz, g.B))]
+
+
+@pytest.mark.parametrize("storage,name", [(_lib.F16, "fp16"), (_lib.BF16, "bf16")])
+def test_engine_low_precision_storage_cfg1(cfg1, storage, name):
+    g, sino = cfg1
+    vol, _, _ = run_engine(g, sino.astype(np.float32), storage=storage)
+    ref64 = O.fdk(ogeom(g), sino, kind=0, f64=True)
+    e = O.rel_rmse(vol, ref64)
+    say(f"  {name} storage: rel RMSE vs FP64 oracle {e:.3e}")
+    assert e <= TOL_F16
+
+
+# ------------------------------------------------------------------ odd geometry
+ODD = dict(Nu=37, Nv=23, Nang=7, Nx=19, Ny=23, Nz=17, D=60.0, tau=0.1, scale=1.3, B=16,
+           cX=17.2, cZ=11.7, rot=0.3, flips=(1, 0, 1))
+
+
+def test_engine_odd_geometry_and_dropped_slab():
+    # (Nz-1) % B == 0: the reference would skip slice 16; every slice is produced
+    g = scan(**ODD)
+    rng = np.random.default_rng(1)
+    sino = rng.standard_normal((g.Nv, g.Nang, g.Nu)).astype(np.float32)
+    vol, _, sink = run_engine(g, sino)
+    ref = O.fdk(ogeom(g), sino, kind=0)
+    e = O.rel_rmse(vol, ref)
+    say(f"  odd geometry rel RMSE {e:.3e}; slabs {sink.calls}")
+    assert e <= TOL_FDK
+    assert [c[0] for c in sink.calls] == [0, 1]
+    assert O.slab_count_reference(g.Nz, g.B) == 1
+    assert np.abs(vol[16]).max() > 0
+    assert O.rel_rmse(vol[16], ref[16]) <= TOL_FDK
+
+
+def test_engine_block_size_invariance(cfg1):
+    g, sino = cfg1
+    g = scan(64, 64, 30, 64, 64, 40, B=16)
+    s = O.project_ellipsoids(ogeom(g), O.shepp_logan(32.0)).astype(np.float32)
+    vols = []
+    for B in (1, 7, 16, 39, 64):
+        g.B = B
+        v, _, _ = run_engine(g, s)
+        vols.append(v)
+    for v in vols[1:]:
+        assert np.array_equal(v, vols[0])
+
+
+def test_engine_collecting_two_angle_blocks():
+    # two "sin blocks" (para_TC.cpp:117-155): angles [0,N/2) then [N/2,N) with the
+    # rotation offset, the second accumulated through the slab callback
+    N = 40
+    g = scan(48, 40, N, 48, 48, 33, B=10)
+    s = O.project_ellipsoids(ogeom(g), O.shepp_logan(24.0)).astype(np.float32)
+    full, _, _ = run_engine(g, s)
+    sink = VolumeSink(g.Nx, g.Ny, g.Nz, g.B)
+    for blk in range(2):
+        gb = scan(48, 40, N // 2, 48, 48, 33, B=10, rot=blk * (N // 2) * g.dbeta)
+        gb.dbeta = g.dbeta
+        part = np.ascontiguousarray(s[:, blk * N // 2:(blk + 1) * N // 2, :])
+        fbp = engine(gb, part, sink, collecting=bool(blk))
+        _lib.check(fbp.Initialize(), "Initialize")
+        _lib.check(fbp.CalculateReconstruction(), "CalculateReconstruction")
+        fbp.close()
+    assert sink.calls[-1] == (3, True)
+    # the FFT pads to p of the block's own detector, identical here; only the
+    # float rounding of the angle offsets differs
+    e = O.rel_rmse(sink.volume, full)
+    say(f"  two-block accumulation vs one pass rel RMSE {e:.3e}")
+    assert e <= 1e-6
+
+
+def test_engine_errors_are_reported():
+    g = scan(16, 16, 4, 16, 16, 16, D=4.0)        # D < Nu/2: invalid
+    fbp = engine(g, np.zeros((16, 4, 16), np.float32))
+    rc = fbp.Initialize()
+    assert rc == _lib.FDK_ERR_INVALID_ARGUMENT
+    assert "source distance" in _lib.last_error()
+    fbp.close()
+    fbp = engine(scan(16, 16, 4, 16, 16, 16))
+    assert fbp.CalculateReconstruction() == _lib.FDK_ERR_NOT_INITIALIZED
+    fbp.close()
+
+
+# ------------------------------------------------------------------ stage level
+def test_weight_kernel_matches_mirror(torch):
+    g = scan(**ODD)
+    st = stages(g)
+    rng = np.random.default_rng(2)
+    x = rng.standard_normal((g.Nv, g.Nang, g.Nu)).astype(np.float32)
+    d = torch.from_numpy(x).cuda()
+    st.weight(d)
+    ref = O.weight(ogeom(g), x)
+    u = ulp_diff(d.cpu().numpy(), ref)
+    say(f"  weight max ulp difference {u.max()}")
+    assert u.max() <= 1
+    d64 = torch.from_numpy(x.astype(np.float64)).cuda()
+    st.weight(d64)
+    assert np.allclose(d64.cpu().numpy(), O.weight(ogeom(g), x.astype(np.float64), f64=True),
+                       rtol=1e-15, atol=0)
+
+
+@pytest.mark.parametrize("Nu", [1, 2, 3, 5, 9, 17, 37, 64, 100, 200, 300, 600, 1024, 1536, 2048])
+def test_filter_kernel_all_lengths(torch, Nu):
+    g = scan(Nu, 3, 5, 8, 8, 8, D=max(2500.0, Nu))
+    for filt in (0, 1):
+        st = stages(g, filt=filt)
+        rng = np.random.default_rng(Nu)
+        x = rng.standard_normal((g.Nv, g.Nang, g.Nu))
+        Hf = O.filter_spectrum(filt, Nu, g.tau)
+        ref = O.filter_rows(ogeom(g), x, Hf, f64=True)
+        d = torch.from_numpy(x.astype(np.float32)).cuda()
+        maxabs = torch.zeros(g.Nang, dtype=torch.int32, device="cuda")
+        st.filter(d, maxabs=maxabs)
+        y = d.cpu().numpy()
+        e = O.rel_rmse(y, ref)
+        assert e <= 2e-6, (Nu, filt, e)
+        m = maxabs.cpu().numpy().view(np.float32)
+        assert np.allclose(m, np.abs(y).max(axis=(0, 2)), rtol=0, atol=0)
+        if g.p <= 4096:
+            d64 = torch.from_numpy(x).cuda()
+            st.filter(d64)
+            assert O.rel_rmse(d64.cpu().numpy(), ref) <= 1e-13, (Nu, filt)
+
+
+def test_filter_fused_weight_and_user_spectrum(torch):
+    g = scan(37, 6, 9, 8, 8, 8)
+    rng = np.random.default_rng(4)
+    h = rng.standard_normal(g.p)
+    st = stages(g, taps=h)
+    x = rng.standard_normal((g.Nv, g.Nang, g.Nu))
+    re, im = O.spectrum_complex(h)
+    w = O.weight(ogeom(g), x, f64=True)
+    ref = O.filter_rows_complex(ogeom(g), w, re, im, f64=True)
+    d = torch.from_numpy(x).cuda()
+    st.filter(d, weight=True)
+    assert O.rel_rmse(d.cpu().numpy(), ref) <= 1e-13
+    # adjoint flag multiplies by conj(H): <F a, b> = <a, F^T b>
+    a = torch.from_numpy(rng.standard_normal((g.Nv, g.Nang, g.Nu))).cuda()
+    b = torch.from_numpy(rng.standard_normal((g.Nv, g.Nang, g.Nu))).cuda()
+    Fa = st.filter(a.clone())
+    Ftb = st.filter(b.clone(), adjoint=True)
+    lhs = float((Fa * b).sum())
+    rhs = float((a * Ftb).sum())
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+
+
+@pytest.mark.parametrize("geo", ["odd", "cfg1"])
+def test_backproject_kernel_matches_mirror(torch, geo):
+    g = scan(**ODD) if geo == "odd" else geometry.workloads()["cfg1"].geom
+    st = stages(g)
+    rng = np.random.default_rng(5)
+    q = rng.standard_normal((g.Nv, g.Nang, g.Nu))
+    ref32 = O.backproject(ogeom(g), q.astype(np.float32))
+    ref64 = O.backproject(ogeom(g), q, f64=True)
+    d = torch.from_numpy(q.astype(np.float32)).cuda()
+    cells, scales = st.pack(d, _lib.F32)
+    vol = st.backproject(cells, _lib.F32, scales).cpu().numpy()
+    e32, e64 = O.rel_rmse(vol, ref32), O.rel_rmse(vol, ref64)
+    say(f"  BP {geo}: vs mirror {e32:.3e}, vs FP64 {e64:.3e}")
+    assert e32 <= 1e-6 and e64 <= 1e-6
+    d64 = torch.from_numpy(q).cuda()
+    c64, s64 = st.pack(d64, _lib.F64)
+    v64 = st.backproject(c64, _lib.F64, s64).cpu().numpy()
+    assert O.rel_rmse(v64, ref64) <= 1e-12
+    # accumulate flag and angle sub-ranges: two halves add up to the whole
+    h = g.Nang // 2
+    c1, s1 = st.pack(torch.from_numpy(np.ascontiguousarray(q[:, :h]).astype(np.float32)).cuda(), _lib.F32)
+    c2, s2 = st.pack(torch.from_numpy(np.ascontiguousarray(q[:, h:]).astype(np.float32)).cuda(), _lib.F32)
+    v = st.backproject(c1, _lib.F32, s1, ang0=0, nang=h)
+    st.backproject(c2, _lib.F32, s2, ang0=h, nang=g.Nang - h, vol=v, accumulate=True)
+    assert O.rel_rmse(v.cpu().numpy(), ref64) <= 1e-6
+
+
+def test_fp16_scale_and_bf16_storage(torch):
+    g = scan(**ODD)
+    st = stages(g)
+    rng = np.random.default_rng(6)
+    q = (rng.standard_normal((g.Nv, g.Nang, g.Nu)) * 1e4).astype(np.float32)
+    d = torch.from_numpy(q).cuda()
+    maxabs = torch.from_numpy(np.abs(q).max(axis=(0, 2)).astype(np.float32).view(np.int32)).cuda()
+    ref = O.backproject(ogeom(g), q.astype(np.float64), f64=True)
+    for dt, tol in ((_lib.F16, 1e-3), (_lib.BF16, 4e-3)):
+        cells, scales = st.pack(d, dt, maxabs)
+        if dt == _lib.F16:
+            s = scales.cpu().numpy()
+            assert np.allclose(s, np.abs(q).max(axis=(0, 2)) / 16384.0, rtol=1e-7)
+        v = st.backproject(cells, dt, scales).cpu().numpy()
+        e = O.rel_rmse(v, ref)
+        say(f"  storage {dt}: rel RMSE {e:.3e}")
+        assert e <= tol
+
+
+def test_phantom_projector_matches_oracle(torch, cfg1):
+    g, sino = cfg1
+    st = stages(g)
+    d = st.phantom(phantoms.shepp_logan(g.fov_px / 2))
+    assert np.allclose(d.cpu().numpy(), sino.astype(np.float32), rtol=2e-6, atol=1e-6)
+    g2 = scan(40, 30, 12, 16, 16, 16, D=500.0, tau=0.05)
+    e = phantoms.random_ellipsoids(3, 64, 40.0)
+    st2 = stages(g2)
+    d2 = st2.phantom(e).cpu().numpy()
+    ref = O.project_ellipsoids(ogeom(g2), O.random_ellipsoids(3, 64, 40.0))
+    assert np.allclose(d2, ref, rtol=2e-6, atol=1e-6)
+
+
+def test_poisson_noise_matches_oracle(torch):
+    rng = np.random.default_rng(8)
+    p = rng.uniform(0.0, 6.0, 200000).astype(np.float32)
+    d = torch.from_numpy(p).cuda()
+    from tomokvas_amd import _lib as L
+    L.check(L.load().fdk_poisson_log(d.data_ptr(), d.numel(), 1e5, 1234, None))
+    torch.cuda.synchronize()
+    ref = O.poisson_log(p, 1e5, 1234)
+    same = np.mean(d.cpu().numpy() == ref)
+    say(f"  identical noisy samples: {same:.6f}")
+    assert same >= 0.999
+
+
+# ------------------------------------------------------------------ full size (cfg2)
+def test_cfg2_full_size_properties(torch):
+    """512^3 from 900 x 1024^2, 64 ellipsoids + Poisson, Shepp-Logan filter."""
+    w = geometry.workloads()["cfg2"]
+    g = w.geom
+    say("cfg2 full size: generate, filter, backproject")
+    st = stages(g, filt=w.filter)
+    sino = st.phantom(w.ellipsoids())
+    st.poisson_log(sino, w.I0, w.seed)
+    raw = sino.clone()
+    maxabs = torch.zeros(g.Nang, dtype=torch.int32, device="cuda")
+    st.filter(sino, weight=True, maxabs=maxabs)
+    cells, scales = st.pack(sino, _lib.F32)
+    v32 = st.backproject(cells, _lib.F32, scales)
+    c16, s16 = st.pack(sino, _lib.F16, maxabs)
+    v16 = st.backproject(c16, _lib.F16, s16)
+    torch.cuda.synchronize()
+    e16 = O.rel_rmse(v16.cpu().numpy(), v32.cpu().numpy())
+    say(f"  FP16 vs FP32 rel RMSE {e16:.3e}")
+    assert e16 <= TOL_F16
+    # filter: 256 random rows against the FP64 oracle (rows are independent)
+    rng = np.random.default_rng(10)
+    rows = rng.choice(g.Nv * g.Nang, 256, replace=False)
+    flat_in = raw.view(-1, g.Nu)[torch.from_numpy(rows).cuda()].cpu().numpy().astype(np.float64)
+    flat_out = sino.view(-1, g.Nu)[torch.from_numpy(rows).cuda()].cpu().numpy()
+    vs = rows // g.Nang
+    gr = ogeom(g)
+    Hf = O.filter_spectrum(w.filter, g.Nu, g.tau)
+    ref_rows = np.zeros_like(flat_in)
+    for i, v in enumerate(vs):
+        one = O.Geometry(**{**g.oracle_dict(), "Nv": 1, "Nang": 1, "cZ": g.cZ - v})
+        wrow = O.weight(one, flat_in[i].reshape(1, 1, -1), f64=True)
+        ref_rows[i] = O.filter_rows(one, wrow, Hf, f64=True).ravel()
+    ef = O.rel_rmse(flat_out, ref_rows)
+    say(f"  filter rows vs FP64 oracle rel RMSE {ef:.3e}")
+    assert ef <= 2e-6
+    # backprojection: 2000 random voxels against the FP64 oracle on the same input
+    host_f = sino.cpu().numpy().astype(np.float64)
+    pts = np.stack([rng.integers(0, g.Nx, 2000), rng.integers(0, g.Ny, 2000),
+                    rng.integers(0, g.Nz, 2000)], axis=1).astype(np.int32)
+    ref_pts = O.backproject_points(gr, host_f, pts, f64=True)
+    got = v32.cpu().numpy()[pts[:, 2], pts[:, 1], pts[:, 0]]
+    eb = O.rel_rmse(got, ref_pts)
+    say(f"  BP voxel sample vs FP64 oracle rel RMSE {eb:.3e}")
+    assert eb <= 1e-5

@@ -1,0 +1,821 @@
+// fdk_engine.cpp -- the FDK engine behind the C ABI of include/tomokvas/fdk.h.
+//
+// Replaces class FilteredBackProjection (FBPG/kernel.cuh:82-151,
+// FBPG/kernel.cu:218-911).  Differences by design, all MI355X-first:
+//  * the projection stack is ONE HBM allocation (the reference splits it into
+//    at most 10 pitched chunks of AllocationLimit bytes, kernel.cu:342-371);
+//  * the whole volume is reconstructed in one launch and then handed to the
+//    slab callback slab by slab (the reference launches once per Z-slab,
+//    kernel.cu:819-888), so slab size never limits the kernel's parallelism;
+//  * geometry tables and the filter spectrum are built in FP64 on the host;
+//  * the filtered stack is repacked into 2x2 bilinear cells, in FP32, FP16
+//    (per-projection scale) or BF16;
+//  * destroy frees only this context's memory (no device reset).
+#include "fdk_internal.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <iostream>
+
+namespace tkv {
+const char *last_error();
+}
+
+using tkv::fail;
+
+struct fdk_ctx {
+    int dev = 0;
+    tkv::Geometry g;
+    int collecting = 0, show = 0;
+    long long alloc_limit = 1024LL * 1024 * 1024;
+    int filter_kind = FDK_FILTER_RAMLAK;
+    std::vector<double> user_taps;
+    int storage = FDK_F32;
+    const float *host_sino = nullptr;
+    bool have_sino = false, have_rec = false;
+
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[5] = {};
+    float stage_ms[4] = {0, 0, 0, 0};
+
+    // tables (prepared)
+    bool prepared = false;
+    int p = 0, nz_pad = 0;
+    float *d_ang32 = nullptr, *d_ax32 = nullptr, *d_ay32 = nullptr, *d_az32 = nullptr;
+    double *d_ang64 = nullptr, *d_ax64 = nullptr, *d_ay64 = nullptr, *d_az64 = nullptr;
+    float *d_H32re = nullptr, *d_H32im = nullptr, *d_tw32 = nullptr, *d_one32 = nullptr;
+    double *d_H64re = nullptr, *d_H64im = nullptr, *d_tw64 = nullptr, *d_one64 = nullptr;
+    bool complex_H = false;
+
+    // engine buffers (initialized)
+    bool initialized = false;
+    float *d_sino = nullptr, *d_filt = nullptr, *d_vol = nullptr, *d_partial = nullptr;
+    void *d_cells = nullptr;
+    float *d_scales = nullptr;
+    unsigned *d_maxabs = nullptr;
+    void *d_scratch = nullptr;
+    size_t scratch_bytes = 0;
+};
+
+namespace {
+
+template <typename T>
+void dfree(T *&p)
+{
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+void free_tables(fdk_ctx *c)
+{
+    dfree(c->d_ang32); dfree(c->d_ax32); dfree(c->d_ay32); dfree(c->d_az32);
+    dfree(c->d_ang64); dfree(c->d_ax64); dfree(c->d_ay64); dfree(c->d_az64);
+    dfree(c->d_H32re); dfree(c->d_H32im); dfree(c->d_tw32); dfree(c->d_one32);
+    dfree(c->d_H64re); dfree(c->d_H64im); dfree(c->d_tw64); dfree(c->d_one64);
+    c->prepared = false;
+}
+
+void free_engine(fdk_ctx *c)
+{
+    dfree(c->d_sino); dfree(c->d_filt); dfree(c->d_vol); dfree(c->d_partial);
+    dfree(c->d_cells); dfree(c->d_scales); dfree(c->d_maxabs); dfree(c->d_scratch);
+    c->scratch_bytes = 0;
+    c->initialized = false;
+}
+
+template <typename T>
+int upload(T *&d, const std::vector<T> &h)
+{
+    dfree(d);
+    TKV_HIP(hipMalloc((void **)&d, sizeof(T) * std::max<size_t>(h.size(), 1)));
+    if (!h.empty()) TKV_HIP(hipMemcpy(d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
+    return FDK_OK;
+}
+
+int dalloc(void **p, size_t bytes, const char *what)
+{
+    hipError_t e = hipMalloc(p, bytes ? bytes : 1);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        return fail(e == hipErrorOutOfMemory ? FDK_ERR_OUT_OF_MEMORY : FDK_ERR_HIP,
+                    std::string(what) + ": hipMalloc of " + std::to_string(bytes) +
+                        " bytes failed: " + hipGetErrorString(e));
+    }
+    return FDK_OK;
+}
+
+hipStream_t pick(fdk_ctx *c, void *stream)
+{
+    return stream ? (hipStream_t)stream : c->stream;
+}
+
+int check_geometry(const tkv::Geometry &g)
+{
+    if (g.Nu < 1 || g.Nv < 1 || g.Nang < 1)
+        return fail(FDK_ERR_INVALID_ARGUMENT, "sinogram sizes must be positive");
+    if (g.Nx < 1 || g.Ny < 1 || g.Nz < 1)
+        return fail(FDK_ERR_INVALID_ARGUMENT, "reconstruction sizes must be positive");
+    if (g.B < 1) return fail(FDK_ERR_INVALID_ARGUMENT, "block size must be positive");
+    if (!(g.tau > 0)) return fail(FDK_ERR_INVALID_ARGUMENT, "sampling must be positive");
+    if (!(g.D > 0.5f * g.Nu))
+        return fail(FDK_ERR_INVALID_ARGUMENT,
+                    "source distance must exceed Nu/2 (the field-of-view radius)");
+    if (tkv::padded_length(g.Nu) > 8192)
+        return fail(FDK_ERR_UNSUPPORTED, "detector rows longer than 4096 are not supported");
+    if (!(g.scale > 0)) return fail(FDK_ERR_INVALID_ARGUMENT, "scale must be positive");
+    return FDK_OK;
+}
+
+int set_device(fdk_ctx *c)
+{
+    TKV_HIP(hipSetDevice(c->dev));
+    return FDK_OK;
+}
+
+int prepare(fdk_ctx *c)
+{
+    if (!c->have_sino || !c->have_rec)
+        return fail(FDK_ERR_NOT_INITIALIZED,
+                    "set sinogram and reconstruction properties before preparing");
+    int rc = check_geometry(c->g);
+    if (rc) return rc;
+    if ((rc = set_device(c))) return rc;
+    free_tables(c);
+    const tkv::Geometry &g = c->g;
+    c->p = tkv::padded_length(g.Nu);
+    // z tables cover every z-run of every kernel (largest run 16) plus slack
+    c->nz_pad = ((g.Nz + 15) / 16) * 16 + 16;
+
+    std::vector<double> s, co;
+    tkv::angle_tables(g, s, co);
+    std::vector<double> a64(2 * g.Nang);
+    std::vector<float> a32(2 * g.Nang);
+    for (int k = 0; k < g.Nang; ++k) {
+        a64[2 * k] = co[k];
+        a64[2 * k + 1] = s[k];
+        a32[2 * k] = (float)co[k];
+        a32[2 * k + 1] = (float)s[k];
+    }
+    if ((rc = upload(c->d_ang32, a32)) || (rc = upload(c->d_ang64, a64))) return rc;
+
+    std::vector<float> ax, ay, az;
+    tkv::axis_tables_f32(g, c->nz_pad, ax, ay, az);
+    if ((rc = upload(c->d_ax32, ax)) || (rc = upload(c->d_ay32, ay)) ||
+        (rc = upload(c->d_az32, az)))
+        return rc;
+    std::vector<double> bx, by, bz;
+    tkv::axis_tables_f64(g, c->nz_pad, bx, by, bz);
+    if ((rc = upload(c->d_ax64, bx)) || (rc = upload(c->d_ay64, by)) ||
+        (rc = upload(c->d_az64, bz)))
+        return rc;
+
+    if (c->filter_kind == FDK_FILTER_USER && (int)c->user_taps.size() != c->p)
+        return fail(FDK_ERR_INVALID_ARGUMENT,
+                    "user filter needs p = " + std::to_string(c->p) + " taps, got " +
+                        std::to_string(c->user_taps.size()));
+    std::vector<double> Hre, Him;
+    tkv::build_spectrum(c->filter_kind, g, c->user_taps, Hre, Him);
+    c->complex_H = c->filter_kind == FDK_FILTER_USER;
+    std::vector<float> H32re(Hre.begin(), Hre.end()), H32im(Him.begin(), Him.end());
+    if ((rc = upload(c->d_H32re, H32re)) || (rc = upload(c->d_H32im, H32im)) ||
+        (rc = upload(c->d_H64re, Hre)) || (rc = upload(c->d_H64im, Him)))
+        return rc;
+
+    // twiddles exp(-2 pi i j / p), formed in double from the exact index
+    std::vector<double> tw64(2 * c->p);
+    std::vector<float> tw32(2 * c->p);
+    for (int j = 0; j < c->p; ++j) {
+        double ang = 2.0 * M_PI * (double)j / (double)c->p;
+        tw64[2 * j] = std::cos(ang);
+        tw64[2 * j + 1] = -std::sin(ang);
+        tw32[2 * j] = (float)tw64[2 * j];
+        tw32[2 * j + 1] = (float)tw64[2 * j + 1];
+    }
+    if ((rc = upload(c->d_tw32, tw32)) || (rc = upload(c->d_tw64, tw64))) return rc;
+    std::vector<float> one32(g.Nang, 1.0f);
+    std::vector<double> one64(g.Nang, 1.0);
+    if ((rc = upload(c->d_one32, one32)) || (rc = upload(c->d_one64, one64))) return rc;
+    c->prepared = true;
+    return FDK_OK;
+}
+
+int need_prepared(fdk_ctx *c)
+{
+    if (!c) return fail(FDK_ERR_INVALID_ARGUMENT, "null context");
+    if (!c->prepared) return fail(FDK_ERR_NOT_INITIALIZED, "call fdk_prepare or fdk_initialize first");
+    return set_device(c);
+}
+
+int cell_bytes(int dtype)
+{
+    switch (dtype) {
+    case FDK_F32: return 8;
+    case FDK_F16: return 4;
+    case FDK_BF16: return 4;
+    case FDK_F64: return 16;
+    default: return 0;
+    }
+}
+
+// cell rows are padded to a multiple of 64 cells with at least one zero
+// column (the right-hand tap of u = Nu-1 has weight 0 but is loaded)
+int cell_pitch(int Nu) { return ((Nu + 1 + 63) / 64) * 64; }
+
+double out_scale(const fdk_ctx *c, int dtype)
+{
+    // result *= delta_f * scaling, scaling = 1/p (kernel.cu:113, :575); float
+    // product for the float paths, double for FP64
+    if (dtype == FDK_F64) return (double)c->g.dbeta * (1.0 / (double)c->p);
+    return (double)(c->g.dbeta * (1.0f / (float)c->p));
+}
+
+int stage_filter(fdk_ctx *c, int dtype, const void *in, void *out, int nang, int weight,
+                 int adjoint, unsigned *maxabs, hipStream_t st)
+{
+    tkv::FilterLaunch L;
+    L.in = in;
+    L.out = out;
+    L.Nu = c->g.Nu;
+    L.Nv = c->g.Nv;
+    L.nang = nang;
+    L.p = c->p;
+    L.weight = weight;
+    L.cX = c->g.cX;
+    L.cZ = c->g.cZ;
+    L.D = c->g.D;
+    const bool f64 = dtype == FDK_F64;
+    L.Hre = f64 ? (const void *)c->d_H64re : (const void *)c->d_H32re;
+    L.Him = c->complex_H ? (f64 ? (const void *)c->d_H64im : (const void *)c->d_H32im) : nullptr;
+    L.conj = adjoint;
+    L.tw = f64 ? (const void *)c->d_tw64 : (const void *)c->d_tw32;
+    L.maxabs = maxabs;
+    return tkv::launch_filter(f64 ? 8 : 4, L, st);
+}
+
+int stage_backproject(fdk_ctx *c, int cell_dtype, const void *cells, const void *scales,
+                      int ang0, int nang, void *vol, int accumulate, hipStream_t st)
+{
+    const bool f64 = cell_dtype == FDK_F64;
+    tkv::BPLaunch L;
+    L.cells = cells;
+    L.scales = scales ? scales : (f64 ? (const void *)c->d_one64 : (const void *)c->d_one32);
+    L.ang = f64 ? (const void *)(c->d_ang64 + 2 * ang0) : (const void *)(c->d_ang32 + 2 * ang0);
+    L.ax = f64 ? (const void *)c->d_ax64 : (const void *)c->d_ax32;
+    L.ay = f64 ? (const void *)c->d_ay64 : (const void *)c->d_ay32;
+    L.az = f64 ? (const void *)c->d_az64 : (const void *)c->d_az32;
+    L.vol = vol;
+    L.Nu = c->g.Nu;
+    L.Nv = c->g.Nv;
+    L.nang = nang;
+    L.pitch = cell_pitch(c->g.Nu);
+    L.Nx = c->g.Nx;
+    L.Ny = c->g.Ny;
+    L.Nz = c->g.Nz;
+    L.cX = c->g.cX;
+    L.cZ = c->g.cZ;
+    L.D = c->g.D;
+    L.out_scale = out_scale(c, cell_dtype);
+    L.maskR2 = 0.25 * (double)c->g.Nu * (double)c->g.Nu;
+    L.accumulate = accumulate;
+    return tkv::launch_backproject(cell_dtype, L, st);
+}
+
+int check_angles(fdk_ctx *c, int ang0, int nang)
+{
+    if (ang0 < 0 || nang < 0 || ang0 + nang > c->g.Nang)
+        return fail(FDK_ERR_INVALID_ARGUMENT, "angle range outside [0, Nang)");
+    return FDK_OK;
+}
+
+}  // namespace
+
+// ====================================================================== C ABI
+extern "C" {
+
+const char *fdk_last_error_string(void) { return tkv::last_error(); }
+
+int fdk_create(int gpu_id, fdk_ctx **out)
+{
+    if (!out) return fail(FDK_ERR_INVALID_ARGUMENT, "null output pointer");
+    *out = nullptr;
+    int n = 0;
+    TKV_HIP(hipGetDeviceCount(&n));
+    if (gpu_id < 0 || gpu_id >= n)
+        return fail(FDK_ERR_INVALID_ARGUMENT, "GPU id " + std::to_string(gpu_id) +
+                                                  " out of range (" + std::to_string(n) +
+                                                  " devices)");
+    fdk_ctx *c = new fdk_ctx();
+    c->dev = gpu_id;
+    hipError_t e = hipSetDevice(gpu_id);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    for (int i = 0; i < 5 && e == hipSuccess; ++i) e = hipEventCreate(&c->ev[i]);
+    if (e != hipSuccess) {
+        delete c;
+        return tkv::hip_fail(e, "fdk_create");
+    }
+    *out = c;
+    return FDK_OK;
+}
+
+int fdk_destroy(fdk_ctx *c)
+{
+    if (!c) return FDK_OK;
+    (void)hipSetDevice(c->dev);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    free_engine(c);
+    free_tables(c);
+    for (auto &e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return FDK_OK;
+}
+
+size_t fdk_get_free_mem_info(int gpu_id)
+{
+    int cur = 0;
+    if (hipGetDevice(&cur) != hipSuccess) return 0;
+    if (hipSetDevice(gpu_id) != hipSuccess) return 0;
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) fr = 0;
+    (void)hipSetDevice(cur);
+    return fr;
+}
+
+int fdk_set_gpu_id(fdk_ctx *c, int gpu_id)
+{
+    if (!c) return fail(FDK_ERR_INVALID_ARGUMENT, "null context");
+    if (c->prepared || c->initialized)
+        return fail(FDK_ERR_INVALID_ARGUMENT, "GPU id must be set before initialisation");
+    int n = 0;
+    TKV_HIP(hipGetDeviceCount(&n));
+    if (gpu_id < 0 || gpu_id >= n) return fail(FDK_ERR_INVALID_ARGUMENT, "GPU id out of range");
+    // the stream and events belong to the creation device: recreate them
+    (void)hipSetDevice(c->dev);
+    for (auto &e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    c->dev = gpu_id;
+    TKV_HIP(hipSetDevice(gpu_id));
+    TKV_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    for (auto &e : c->ev) TKV_HIP(hipEventCreate(&e));
+    return FDK_OK;
+}
+
+int fdk_set_block_size(fdk_ctx *c, int slices)
+{
+    if (!c || slices < 1) return fail(FDK_ERR_INVALID_ARGUMENT, "block size must be positive");
+    c->g.B = slices;
+    c->prepared = false;
+    return FDK_OK;
+}
+
+int fdk_set_allocation_limit(fdk_ctx *c, long long bytes)
+{
+    if (!c || bytes <= 0) return fail(FDK_ERR_INVALID_ARGUMENT, "allocation limit must be positive");
+    c->alloc_limit = bytes;
+    return FDK_OK;
+}
+
+int fdk_set_collecting(fdk_ctx *c, int collecting)
+{
+    if (!c) return fail(FDK_ERR_INVALID_ARGUMENT, "null context");
+    c->collecting = collecting ? 1 : 0;
+    return FDK_OK;
+}
+
+int fdk_set_show_info(fdk_ctx *c, int show)
+{
+    if (!c) return fail(FDK_ERR_INVALID_ARGUMENT, "null context");
+    c->show = show ? 1 : 0;
+    return FDK_OK;
+}
+
+// SetAxisDirection (kernel.cu:499-507): the reference only ever sets a flip
+// (calling it twice cannot clear one); here each call states all three.
+int fdk_set_axis_direction(fdk_ctx *c, int fx, int fy, int fz)
+{
+    if (!c) return fail(FDK_ERR_INVALID_ARGUMENT, "null context");
+    c->g.flipX = fx ? 1 : 0;
+    c->g.flipY = fy ? 1 : 0;
+    c->g.flipZ = fz ? 1 : 0;
+    c->prepared = false;
+    return FDK_OK;
+}
+
+int fdk_set_sinogram_properties(fdk_ctx *c, const float *sino, int Nu, float cX, int Nang,
+                                int Nv, float cZ, float dbeta, float distance, float sampling)
+{
+    if (!c) return fail(FDK_ERR_INVALID_ARGUMENT, "null context");
+    c->host_sino = sino;
+    c->g.Nu = Nu;
+    c->g.cX = cX;
+    c->g.Nang = Nang;
+    c->g.Nv = Nv;
+    c->g.cZ = cZ;
+    c->g.dbeta = dbeta;
+    c->g.D = distance;
+    c->g.tau = sampling;
+    c->have_sino = true;
+    c->prepared = false;
+    return FDK_OK;
+}
+
+int fdk_set_reconstruction_properties(fdk_ctx *c, int Nx, float cx, int Ny, float cy, int Nz,
+                                      float cz, float rot)
+{
+    if (!c) return fail(FDK_ERR_INVALID_ARGUMENT, "null context");
+    c->g.Nx = Nx;
+    c->g.cx = cx;
+    c->g.Ny = Ny;
+    c->g.cy = cy;
+    c->g.Nz = Nz;
+    c->g.cz = cz;
+    c->g.rot = rot;
+    c->have_rec = true;
+    c->prepared = false;
+    return FDK_OK;
+}
+
+int fdk_set_scale_xyz(fdk_ctx *c, float scale)
+{
+    if (!c) return fail(FDK_ERR_INVALID_ARGUMENT, "null context");
+    c->g.scale = scale;
+    c->prepared = false;
+    return FDK_OK;
+}
+
+int fdk_set_filter(fdk_ctx *c, int kind, const double *taps, int ntaps)
+{
+    if (!c) return fail(FDK_ERR_INVALID_ARGUMENT, "null context");
+    if (kind != FDK_FILTER_RAMLAK && kind != FDK_FILTER_SHEPPLOGAN && kind != FDK_FILTER_USER)
+        return fail(FDK_ERR_INVALID_ARGUMENT, "unknown filter kind");
+    if (kind == FDK_FILTER_USER && (!taps || ntaps < 1))
+        return fail(FDK_ERR_INVALID_ARGUMENT, "user filter needs taps");
+    c->filter_kind = kind;
+    c->user_taps.assign(taps && kind == FDK_FILTER_USER ? taps : nullptr,
+                        taps && kind == FDK_FILTER_USER ? taps + ntaps : nullptr);
+    c->prepared = false;
+    return FDK_OK;
+}
+
+int fdk_set_storage(fdk_ctx *c, int dtype)
+{
+    if (!c) return fail(FDK_ERR_INVALID_ARGUMENT, "null context");
+    if (dtype != FDK_F32 && dtype != FDK_F16 && dtype != FDK_BF16)
+        return fail(FDK_ERR_INVALID_ARGUMENT, "storage must be F32, F16 or BF16");
+    if (c->initialized && dtype != c->storage) {
+        dfree(c->d_cells);
+        size_t bytes = fdk_cells_bytes(c, dtype, c->g.Nang);
+        int rc = dalloc(&c->d_cells, bytes, "cell stack");
+        if (rc) return rc;
+    }
+    c->storage = dtype;
+    return FDK_OK;
+}
+
+int fdk_prepare(fdk_ctx *c)
+{
+    if (!c) return fail(FDK_ERR_INVALID_ARGUMENT, "null context");
+    return prepare(c);
+}
+
+size_t fdk_cells_bytes(fdk_ctx *c, int cell_dtype, int nang)
+{
+    if (!c) return 0;
+    return (size_t)cell_bytes(cell_dtype) * (size_t)cell_pitch(c->g.Nu) * (size_t)c->g.Nv *
+           (size_t)nang;
+}
+
+int fdk_initialize(fdk_ctx *c)
+{
+    if (!c) return fail(FDK_ERR_INVALID_ARGUMENT, "null context");
+    int rc = prepare(c);
+    if (rc) return rc;
+    free_engine(c);
+    const tkv::Geometry &g = c->g;
+    const size_t nsino = (size_t)g.Nu * g.Nv * g.Nang;
+    const size_t nvol = (size_t)g.Nx * g.Ny * g.Nz;
+    const size_t slab = (size_t)g.Nx * g.Ny * g.B;
+    if ((rc = dalloc((void **)&c->d_sino, nsino * sizeof(float), "projection stack")) ||
+        (rc = dalloc((void **)&c->d_filt, nsino * sizeof(float), "filtered stack")) ||
+        (rc = dalloc(&c->d_cells, fdk_cells_bytes(c, c->storage, g.Nang), "cell stack")) ||
+        (rc = dalloc((void **)&c->d_vol, nvol * sizeof(float), "volume")) ||
+        (rc = dalloc((void **)&c->d_scales, g.Nang * sizeof(float), "scales")) ||
+        (rc = dalloc((void **)&c->d_maxabs, g.Nang * sizeof(unsigned), "maxabs"))) {
+        free_engine(c);
+        return rc;
+    }
+    if (c->collecting &&
+        (rc = dalloc((void **)&c->d_partial, slab * sizeof(float), "collect slab"))) {
+        free_engine(c);
+        return rc;
+    }
+    if (c->host_sino) {
+        hipError_t e = hipMemcpy(c->d_sino, c->host_sino, nsino * sizeof(float),
+                                 hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            free_engine(c);
+            return tkv::hip_fail(e, "projection upload");
+        }
+    }
+    c->host_sino = nullptr;  // borrowed only until here (para_TC.cpp:150 frees it)
+    c->initialized = true;
+    if (c->show) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, c->dev) == hipSuccess)
+            std::cout << "GPU " << c->dev << ": " << prop.name << ", "
+                      << prop.multiProcessorCount << " CUs, "
+                      << prop.totalGlobalMem / (1024 * 1024) << " MiB\n";
+    }
+    return FDK_OK;
+}
+
+float *fdk_device_sinogram(fdk_ctx *c) { return c && c->initialized ? c->d_sino : nullptr; }
+const float *fdk_device_volume(fdk_ctx *c) { return c && c->initialized ? c->d_vol : nullptr; }
+
+int fdk_calculate_reconstruction(fdk_ctx *c, fdk_get_block_fn get_block, void *user)
+{
+    if (!c) return fail(FDK_ERR_INVALID_ARGUMENT, "null context");
+    if (!c->initialized) return fail(FDK_ERR_NOT_INITIALIZED, "call fdk_initialize first");
+    int rc = set_device(c);
+    if (rc) return rc;
+    const tkv::Geometry &g = c->g;
+    hipStream_t st = c->stream;
+    if ((rc = tkv::launch_zero_u32(c->d_maxabs, g.Nang, st))) return rc;
+    TKV_HIP(hipEventRecord(c->ev[0], st));
+    // cosine weight fused into the row filter (kernel.cu:807-808)
+    if ((rc = stage_filter(c, FDK_F32, c->d_sino, c->d_filt, g.Nang, 1, 0, c->d_maxabs, st)))
+        return rc;
+    TKV_HIP(hipEventRecord(c->ev[1], st));
+    if ((rc = tkv::launch_pack(FDK_F32, c->d_filt, g.Nu, g.Nv, g.Nang, cell_pitch(g.Nu),
+                               c->d_maxabs, c->storage, c->d_cells, c->d_scales, st)))
+        return rc;
+    TKV_HIP(hipEventRecord(c->ev[2], st));
+    if ((rc = stage_backproject(c, c->storage, c->d_cells, c->d_scales, 0, g.Nang, c->d_vol, 0,
+                                st)))
+        return rc;
+    TKV_HIP(hipEventRecord(c->ev[3], st));
+    // hand the volume out slab by slab (kernel.cu:838-884)
+    if (get_block) {
+        const int nslab = tkv::slab_count(g.Nz, g.B);
+        const size_t plane = (size_t)g.Nx * g.Ny;
+        for (int n = 0; n < nslab; ++n) {
+            const int nz = std::min(g.B, g.Nz - n * g.B);
+            TKV_HIP(hipStreamSynchronize(st));
+            float *block = get_block(user, n, c->collecting);
+            if (!block)
+                return fail(FDK_ERR_CALLBACK, "slab callback returned NULL for slab " +
+                                                  std::to_string(n));
+            float *dslab = c->d_vol + (size_t)n * g.B * plane;
+            if (c->collecting) {
+                TKV_HIP(hipMemcpyAsync(c->d_partial, block, nz * plane * sizeof(float),
+                                       hipMemcpyHostToDevice, st));
+                if ((rc = tkv::launch_accumulate(dslab, c->d_partial, nz * plane, st))) return rc;
+            }
+            TKV_HIP(hipMemcpyAsync(block, dslab, nz * plane * sizeof(float),
+                                   hipMemcpyDeviceToHost, st));
+        }
+    }
+    TKV_HIP(hipEventRecord(c->ev[4], st));
+    TKV_HIP(hipStreamSynchronize(st));
+    for (int i = 0; i < 4; ++i)
+        TKV_HIP(hipEventElapsedTime(&c->stage_ms[i], c->ev[i], c->ev[i + 1]));
+    return FDK_OK;
+}
+
+int fdk_get_stage_times(fdk_ctx *c, float *ms4)
+{
+    if (!c || !ms4) return fail(FDK_ERR_INVALID_ARGUMENT, "null argument");
+    for (int i = 0; i < 4; ++i) ms4[i] = c->stage_ms[i];
+    return FDK_OK;
+}
+
+int fdk_get_geometry(fdk_ctx *c, int *ints8, float *floats12)
+{
+    if (!c || !ints8 || !floats12) return fail(FDK_ERR_INVALID_ARGUMENT, "null argument");
+    const tkv::Geometry &g = c->g;
+    int iv[8] = {g.Nu, g.Nang, g.Nv, g.Nx, g.Ny, g.Nz, g.B, c->p};
+    float fv[12] = {g.cX, g.cZ, g.dbeta, g.D, g.tau, g.cx, g.cy, g.cz, g.rot, g.scale,
+                    (float)(g.flipX | (g.flipY << 1) | (g.flipZ << 2)), (float)c->storage};
+    std::memcpy(ints8, iv, sizeof iv);
+    std::memcpy(floats12, fv, sizeof fv);
+    return FDK_OK;
+}
+
+// ------------------------------------------------------------ stage level
+int fdk_weight(fdk_ctx *c, int dtype, void *d_sino, int nang, void *stream)
+{
+    int rc = need_prepared(c);
+    if (rc) return rc;
+    if (dtype != FDK_F32 && dtype != FDK_F64)
+        return fail(FDK_ERR_INVALID_ARGUMENT, "weight: F32 or F64 data");
+    if (nang < 0) return fail(FDK_ERR_INVALID_ARGUMENT, "weight: negative angle count");
+    return tkv::launch_weight(dtype == FDK_F64 ? 8 : 4, d_sino, c->g.Nu, c->g.Nv, nang, c->g.cX,
+                              c->g.cZ, c->g.D, pick(c, stream));
+}
+
+int fdk_filter(fdk_ctx *c, int dtype, void *d_sino, int nang, int weight, int adjoint,
+               uint32_t *d_maxabs, void *stream)
+{
+    int rc = need_prepared(c);
+    if (rc) return rc;
+    if (dtype != FDK_F32 && dtype != FDK_F64)
+        return fail(FDK_ERR_INVALID_ARGUMENT, "filter: F32 or F64 data");
+    if (nang < 0) return fail(FDK_ERR_INVALID_ARGUMENT, "filter: negative angle count");
+    return stage_filter(c, dtype, d_sino, d_sino, nang, weight, adjoint, d_maxabs,
+                        pick(c, stream));
+}
+
+int fdk_pack(fdk_ctx *c, int dtype_in, const void *d_filtered, int nang,
+             const uint32_t *d_maxabs, int cell_dtype, void *d_cells, void *d_scales,
+             void *stream)
+{
+    int rc = need_prepared(c);
+    if (rc) return rc;
+    if (!d_filtered || !d_cells || !d_scales)
+        return fail(FDK_ERR_INVALID_ARGUMENT, "pack: null buffer");
+    return tkv::launch_pack(dtype_in, d_filtered, c->g.Nu, c->g.Nv, nang, cell_pitch(c->g.Nu),
+                            d_maxabs, cell_dtype, d_cells, d_scales, pick(c, stream));
+}
+
+int fdk_backproject(fdk_ctx *c, int cell_dtype, const void *d_cells, const void *d_scales,
+                    int ang0, int nang, void *d_vol, int accumulate, void *stream)
+{
+    int rc = need_prepared(c);
+    if (rc) return rc;
+    if ((rc = check_angles(c, ang0, nang))) return rc;
+    if (!d_cells || !d_vol) return fail(FDK_ERR_INVALID_ARGUMENT, "backproject: null buffer");
+    return stage_backproject(c, cell_dtype, d_cells, d_scales, ang0, nang, d_vol, accumulate,
+                             pick(c, stream));
+}
+
+int fdk_forward_project(fdk_ctx *c, int dtype, const void *d_vol, int ang0, int nang,
+                        void *d_sino, void *stream)
+{
+    int rc = need_prepared(c);
+    if (rc) return rc;
+    if ((rc = check_angles(c, ang0, nang))) return rc;
+    if (dtype != FDK_F32 && dtype != FDK_F64)
+        return fail(FDK_ERR_INVALID_ARGUMENT, "forward_project: F32 or F64 data");
+    const bool f64 = dtype == FDK_F64;
+    tkv::FPLaunch L;
+    L.vol = d_vol;
+    L.sino = d_sino;
+    L.ang = f64 ? (const void *)(c->d_ang64 + 2 * ang0) : (const void *)(c->d_ang32 + 2 * ang0);
+    L.ax = f64 ? (const void *)c->d_ax64 : (const void *)c->d_ax32;
+    L.ay = f64 ? (const void *)c->d_ay64 : (const void *)c->d_ay32;
+    L.az = f64 ? (const void *)c->d_az64 : (const void *)c->d_az32;
+    L.Nu = c->g.Nu;
+    L.Nv = c->g.Nv;
+    L.nang = nang;
+    L.Nx = c->g.Nx;
+    L.Ny = c->g.Ny;
+    L.Nz = c->g.Nz;
+    L.cX = c->g.cX;
+    L.cZ = c->g.cZ;
+    L.D = c->g.D;
+    L.out_scale = out_scale(c, dtype);
+    L.maskR2 = 0.25 * (double)c->g.Nu * (double)c->g.Nu;
+    return tkv::launch_forward_project(f64 ? 8 : 4, L, pick(c, stream));
+}
+
+int fdk_tap_gradient(fdk_ctx *c, int dtype, const void *d_q, const void *d_G, int nang,
+                     double *dh, void *stream)
+{
+    int rc = need_prepared(c);
+    if (rc) return rc;
+    if (dtype != FDK_F32 && dtype != FDK_F64)
+        return fail(FDK_ERR_INVALID_ARGUMENT, "tap_gradient: F32 or F64 data");
+    if (!dh) return fail(FDK_ERR_INVALID_ARGUMENT, "tap_gradient: null output");
+    const int p = c->p;
+    const long long rows = (long long)c->g.Nv * nang;
+    const int tpt = std::min(p / 4, 256), nt = 256 / tpt;
+    long long need = ((rows + 1) / 2 + nt - 1) / nt;
+    const int nblocks = (int)std::max<long long>(1, std::min<long long>(need, 2048));
+    const size_t rb = dtype == FDK_F64 ? 8 : 4;
+    const size_t bytes = (size_t)nblocks * p * 2 * rb;
+    if (bytes > c->scratch_bytes) {
+        dfree(c->d_scratch);
+        c->scratch_bytes = 0;
+        if ((rc = dalloc(&c->d_scratch, bytes, "tap gradient partials"))) return rc;
+        c->scratch_bytes = bytes;
+    }
+    hipStream_t st = pick(c, stream);
+    tkv::TapGradLaunch L;
+    L.q = d_q;
+    L.G = d_G;
+    L.Nu = c->g.Nu;
+    L.rows = (int)std::min<long long>(rows, 0x7fffffff);
+    L.p = p;
+    L.tw = dtype == FDK_F64 ? (const void *)c->d_tw64 : (const void *)c->d_tw32;
+    L.partial = c->d_scratch;
+    L.nblocks = nblocks;
+    if ((rc = tkv::launch_tap_gradient((int)rb, L, st))) return rc;
+    std::vector<char> host(bytes);
+    TKV_HIP(hipMemcpyAsync(host.data(), c->d_scratch, bytes, hipMemcpyDeviceToHost, st));
+    TKV_HIP(hipStreamSynchronize(st));
+    // sum the partial spectra in double, take the Hermitian part, final DFT
+    std::vector<double> Pr(p, 0.), Pi(p, 0.);
+    for (int b = 0; b < nblocks; ++b)
+        for (int k = 0; k < p; ++k) {
+            size_t idx = ((size_t)b * p + k) * 2;
+            if (rb == 8) {
+                const double *h = (const double *)host.data();
+                Pr[k] += h[idx];
+                Pi[k] += h[idx + 1];
+            } else {
+                const float *h = (const float *)host.data();
+                Pr[k] += h[idx];
+                Pi[k] += h[idx + 1];
+            }
+        }
+    std::vector<double> Rr(p), Ri(p), cs(p), sn(p);
+    for (int k = 0; k < p; ++k) {
+        int m = (p - k) % p;
+        Rr[k] = 0.5 * (Pr[k] + Pr[m]);
+        Ri[k] = 0.5 * (Pi[k] - Pi[m]);
+        double a = 2.0 * M_PI * (double)k / (double)p;
+        cs[k] = std::cos(a);
+        sn[k] = std::sin(a);
+    }
+    for (int j = 0; j < p; ++j) {
+        double s = 0.;
+        for (int k = 0; k < p; ++k) {
+            int idx = (int)(((long long)k * j) % p);
+            // Re(R[k] e^{-2 pi i k j / p})
+            s += Rr[k] * cs[idx] + Ri[k] * sn[idx];
+        }
+        dh[j] += s;
+    }
+    return FDK_OK;
+}
+
+int fdk_accumulate(float *d_dst, const float *d_src, size_t n, void *stream)
+{
+    if (!d_dst || !d_src) return fail(FDK_ERR_INVALID_ARGUMENT, "accumulate: null buffer");
+    return tkv::launch_accumulate(d_dst, d_src, n, (hipStream_t)stream);
+}
+
+int fdk_compare(const float *d_a, const float *d_b, size_t n, double *out3, void *stream)
+{
+    if (!d_a || !d_b || !out3) return fail(FDK_ERR_INVALID_ARGUMENT, "compare: null buffer");
+    hipStream_t st = (hipStream_t)stream;
+    double *d_out = nullptr;
+    int rc = dalloc((void **)&d_out, 3 * sizeof(double), "compare");
+    if (rc) return rc;
+    rc = tkv::launch_compare(d_a, d_b, n, d_out, st);
+    hipError_t e = rc ? hipSuccess : hipMemcpyAsync(out3, d_out, 3 * sizeof(double),
+                                                    hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess && !rc) e = hipStreamSynchronize(st);
+    (void)hipFree(d_out);
+    if (rc) return rc;
+    if (e != hipSuccess) return tkv::hip_fail(e, "compare");
+    return FDK_OK;
+}
+
+int fdk_phantom_project(fdk_ctx *c, const double *ells, int n, int ang0, int nang,
+                        float *d_sino, void *stream)
+{
+    int rc = need_prepared(c);
+    if (rc) return rc;
+    if ((rc = check_angles(c, ang0, nang))) return rc;
+    if (n < 0 || (n > 0 && !ells) || !d_sino)
+        return fail(FDK_ERR_INVALID_ARGUMENT, "phantom: bad arguments");
+    hipStream_t st = pick(c, stream);
+    double *d_e = nullptr;
+    if ((rc = dalloc((void **)&d_e, sizeof(double) * 19 * std::max(n, 1), "ellipsoids"))) return rc;
+    hipError_t e = hipMemcpyAsync(d_e, ells, sizeof(double) * 19 * n, hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) {
+        (void)hipFree(d_e);
+        return tkv::hip_fail(e, "phantom upload");
+    }
+    tkv::PhantomLaunch L;
+    L.ells = d_e;
+    L.n = n;
+    L.ang = c->d_ang64 + 2 * ang0;
+    L.sino = d_sino;
+    L.Nu = c->g.Nu;
+    L.Nv = c->g.Nv;
+    L.nang = nang;
+    L.ang0_stride = 0;
+    L.cX = c->g.cX;
+    L.cZ = c->g.cZ;
+    L.D = c->g.D;
+    L.tau = c->g.tau;
+    rc = tkv::launch_phantom(L, st);
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(d_e);
+    return rc;
+}
+
+int fdk_poisson_log(float *d_p, size_t n, double I0, uint64_t seed, void *stream)
+{
+    if (!d_p || !(I0 > 0)) return fail(FDK_ERR_INVALID_ARGUMENT, "poisson: bad arguments");
+    return tkv::launch_poisson_log(d_p, n, I0, seed, (hipStream_t)stream);
+}
+
+}  // extern "C"

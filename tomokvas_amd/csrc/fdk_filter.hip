@@ -1,0 +1,637 @@
+// fdk_filter.hip -- cosine pre-weighting, LDS-resident Stockham FFT ramp filter,
+// and the repack of filtered rows into the backprojector's cell stack (gfx950).
+//
+// Reference path: RB_comma3DKernel (FBPG/kernel.cu:179-201), fConvolution with
+// cuFFT R2C/C2R per detector row (FBPG/kernel.cu:557-781), FiltrationKernel
+// (FBPG/kernel.cu:40-53).  Here one workgroup transforms whole rows inside LDS:
+// two real rows travel as the real and imaginary parts of one complex
+// sequence (the spectrum is Hermitian-preserving, so the two filtered rows come
+// back as the real and imaginary parts), zero-padded to p = 2*nextpow2(Nu),
+// forward Stockham FFT (radix-4 passes, one radix-2 pass when log2 p is odd;
+// Van Loan, Computational Frameworks for the FFT, 1992), bin-wise product with
+// the spectrum, inverse FFT without 1/p (the backprojector applies it, as the
+// reference does at FBPG/kernel.cu:575, :113), crop to Nu.  HBM traffic: one
+// read and one write of each detector sample; everything else stays in LDS.
+#include "fdk_internal.hpp"
+
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
+
+namespace tkv {
+
+template <typename R>
+struct cpx {
+    R x, y;
+};
+
+template <typename R>
+__device__ __forceinline__ cpx<R> cmul(cpx<R> a, cpx<R> b)
+{
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+
+// Cosine weight of one detector sample in the reference's float operations
+// (kernel.cu:187, :197-198): D / sqrt(ce^2 + D^2 + cs^2).
+template <typename R>
+__device__ __forceinline__ R cos_weight(int u, int v, R cX, R cZ, R D)
+{
+#pragma clang fp contract(off)
+    R ce = (R)u - cX;
+    R cs = (R)v - cZ;
+    R d2 = D * D;
+    R t = ce * ce;
+    t = t + d2;
+    t = t + cs * cs;
+    return D / sqrt(t);
+}
+
+// ---------------------------------------------------------------- cosine weight
+// Coalesced: a thread per sample, consecutive lanes on consecutive u of one
+// row (the reference walks a whole row per thread, kernel.cu:195-199).
+template <typename R>
+__global__ void __launch_bounds__(256) weight_kernel(R *__restrict__ sino, int Nu, int Nv,
+                                                     int nang, R cX, R cZ, R D)
+{
+    const size_t row = blockIdx.y + (size_t)gridDim.y * blockIdx.z;
+    const int u = blockIdx.x * 256 + threadIdx.x;
+    if (row >= (size_t)Nv * nang || u >= Nu) return;
+    const int v = (int)(row / (size_t)nang);
+    sino[row * Nu + u] *= cos_weight<R>(u, v, cX, cZ, D);
+}
+
+int launch_weight(int real_bytes, void *sino, int Nu, int Nv, int nang, float cX, float cZ,
+                  float D, hipStream_t st)
+{
+    size_t rows = (size_t)Nv * nang;
+    if (rows == 0 || Nu == 0) return FDK_OK;
+    unsigned gy = (unsigned)(rows < 65535 ? rows : 65535);
+    unsigned gz = (unsigned)((rows + gy - 1) / gy);
+    dim3 grid((Nu + 255) / 256, gy, gz);
+    if (real_bytes == 8)
+        hipLaunchKernelGGL(weight_kernel<double>, grid, dim3(256), 0, st, (double *)sino, Nu,
+                           Nv, nang, (double)cX, (double)cZ, (double)D);
+    else
+        hipLaunchKernelGGL(weight_kernel<float>, grid, dim3(256), 0, st, (float *)sino, Nu, Nv,
+                           nang, cX, cZ, D);
+    TKV_HIP(hipGetLastError());
+    return FDK_OK;
+}
+
+// ---------------------------------------------------------------- Stockham FFT
+template <int P>
+struct FFTShape {
+    static constexpr int TPT = (P / 4 < 256) ? P / 4 : 256;  // threads per transform
+    static constexpr int NT = 256 / TPT;                     // transforms per workgroup
+    static constexpr int BPT = (P / 4) / TPT;                // radix-4 butterflies per thread
+    static constexpr int LOG2 = __builtin_ctz(P);
+    static constexpr bool ODD = (LOG2 & 1) != 0;
+};
+
+// One Stockham pass of radix 4 over the transform in `s` (length P).  `pc` is
+// the length of the sub-transforms already done; inputs of butterfly i at
+// i + m*P/4, outputs at ((i - k) << 2) + k + m*pc with k = i mod pc.
+template <typename R, int P, bool INV>
+__device__ __forceinline__ void radix4_pass(cpx<R> *s, const cpx<R> *__restrict__ tw, int t,
+                                            int pc)
+{
+    using S = FFTShape<P>;
+    cpx<R> u[S::BPT][4];
+#pragma unroll
+    for (int b = 0; b < S::BPT; ++b) {
+        int i = t + b * S::TPT;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) u[b][m] = s[i + m * (P / 4)];
+    }
+    __syncthreads();
+    const int stride = P / (4 * pc);
+#pragma unroll
+    for (int b = 0; b < S::BPT; ++b) {
+        int i = t + b * S::TPT;
+        int k = i & (pc - 1);
+        if (pc > 1) {
+#pragma unroll
+            for (int m = 1; m < 4; ++m) {
+                cpx<R> w = tw[m * k * stride];
+                if (INV) w.y = -w.y;
+                u[b][m] = cmul(u[b][m], w);
+            }
+        }
+        cpx<R> v0 = {u[b][0].x + u[b][2].x, u[b][0].y + u[b][2].y};
+        cpx<R> v1 = {u[b][0].x - u[b][2].x, u[b][0].y - u[b][2].y};
+        cpx<R> v2 = {u[b][1].x + u[b][3].x, u[b][1].y + u[b][3].y};
+        cpx<R> d = {u[b][1].x - u[b][3].x, u[b][1].y - u[b][3].y};
+        // forward: d * (-i) = (d.y, -d.x); inverse: d * (+i) = (-d.y, d.x)
+        cpx<R> v3 = INV ? cpx<R>{-d.y, d.x} : cpx<R>{d.y, -d.x};
+        int j = ((i - k) << 2) + k;
+        s[j] = {v0.x + v2.x, v0.y + v2.y};
+        s[j + pc] = {v1.x + v3.x, v1.y + v3.y};
+        s[j + 2 * pc] = {v0.x - v2.x, v0.y - v2.y};
+        s[j + 3 * pc] = {v1.x - v3.x, v1.y - v3.y};
+    }
+    __syncthreads();
+}
+
+// First pass of radix 2 (pc = 1, trivial twiddles): outputs 2i, 2i+1.
+template <typename R, int P>
+__device__ __forceinline__ void radix2_first(cpx<R> *s, int t)
+{
+    using S = FFTShape<P>;
+    constexpr int NB = (P / 2) / S::TPT;
+    cpx<R> u[NB][2];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        int i = t + b * S::TPT;
+        u[b][0] = s[i];
+        u[b][1] = s[i + P / 2];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        int i = t + b * S::TPT;
+        s[2 * i] = {u[b][0].x + u[b][1].x, u[b][0].y + u[b][1].y};
+        s[2 * i + 1] = {u[b][0].x - u[b][1].x, u[b][0].y - u[b][1].y};
+    }
+    __syncthreads();
+}
+
+template <typename R, int P, bool INV>
+__device__ __forceinline__ void fft_lds(cpx<R> *s, const cpx<R> *__restrict__ tw, int t)
+{
+    using S = FFTShape<P>;
+    int pc = 1;
+    if (S::ODD) {
+        radix2_first<R, P>(s, t);
+        pc = 2;
+    }
+#pragma unroll
+    for (int pass = 0; pass < S::LOG2 / 2; ++pass) {
+        radix4_pass<R, P, INV>(s, tw, t, pc);
+        pc *= 4;
+    }
+}
+
+template <typename R>
+__device__ __forceinline__ R group_max(R m, int width)
+{
+    for (int off = 1; off < width && off < 64; off <<= 1) {
+        R o = __shfl_xor(m, off, 64);
+        m = o > m ? o : m;
+    }
+    return m;
+}
+
+__device__ __forceinline__ unsigned float_bits_abs(float m) { return __float_as_uint(m); }
+__device__ __forceinline__ unsigned float_bits_abs(double m) { return __float_as_uint((float)m); }
+
+template <typename R, int P>
+__global__ void __launch_bounds__(256)
+    filter_kernel(const R *in, R *out, int Nu, int Nv, int nang, int weight,
+                  R cX, R cZ, R D, const R *__restrict__ Hre, const R *__restrict__ Him,
+                  int conj, const cpx<R> *__restrict__ tw, unsigned *__restrict__ maxabs)
+{
+    using S = FFTShape<P>;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cpx<R> *smem = reinterpret_cast<cpx<R> *>(smem_raw);
+    const int tid = threadIdx.x;
+    const int f = tid / S::TPT;           // transform within the workgroup
+    const int t = tid - f * S::TPT;       // thread within the transform
+    cpx<R> *s = smem + f * P;
+    const long long rows = (long long)Nv * nang;
+    const long long rA = 2 * ((long long)blockIdx.x * S::NT + f);
+    const long long rB = rA + 1;
+    const bool hasA = rA < rows, hasB = rB < rows;
+    const int vA = hasA ? (int)(rA / nang) : 0;
+    const int vB = hasB ? (int)(rB / nang) : 0;
+    const R *inA = in + (hasA ? rA : 0) * Nu;
+    const R *inB = in + (hasB ? rB : 0) * Nu;
+
+    // load two rows (coalesced along u), optional fused cosine weight, zero pad
+#pragma unroll
+    for (int j = 0; j < P / S::TPT; ++j) {
+        int n = t + j * S::TPT;
+        R a = 0, b = 0;
+        if (n < Nu) {
+            if (hasA) a = inA[n];
+            if (hasB) b = inB[n];
+            if (weight) {
+                a *= cos_weight<R>(n, vA, cX, cZ, D);
+                b *= cos_weight<R>(n, vB, cX, cZ, D);
+            }
+        }
+        s[n] = {a, b};
+    }
+    __syncthreads();
+
+    fft_lds<R, P, false>(s, tw, t);
+
+    // bin-wise product with the spectrum (FiltrationKernel, kernel.cu:40-53)
+#pragma unroll
+    for (int j = 0; j < P / S::TPT; ++j) {
+        int n = t + j * S::TPT;
+        cpx<R> z = s[n];
+        R hr = Hre[n];
+        if (Him) {
+            R hi = conj ? -Him[n] : Him[n];
+            s[n] = {z.x * hr - z.y * hi, z.x * hi + z.y * hr};
+        } else {
+            s[n] = {z.x * hr, z.y * hr};
+        }
+    }
+    __syncthreads();
+
+    fft_lds<R, P, true>(s, tw, t);
+
+    // crop to Nu, write back, per-projection max |value|
+    R mA = 0, mB = 0;
+    R *outA = out + (hasA ? rA : 0) * Nu;
+    R *outB = out + (hasB ? rB : 0) * Nu;
+#pragma unroll
+    for (int j = 0; j < P / S::TPT; ++j) {
+        int n = t + j * S::TPT;
+        if (n < Nu) {
+            cpx<R> z = s[n];
+            if (hasA) outA[n] = z.x;
+            if (hasB) outB[n] = z.y;
+            R ax = z.x < 0 ? -z.x : z.x, ay = z.y < 0 ? -z.y : z.y;
+            mA = ax > mA ? ax : mA;
+            mB = ay > mB ? ay : mB;
+        }
+    }
+    if (maxabs) {
+        mA = group_max(mA, S::TPT);
+        mB = group_max(mB, S::TPT);
+        bool leader = (S::TPT >= 64) ? ((tid & 63) == 0) : (t == 0);
+        if (leader) {
+            if (hasA) atomicMax(&maxabs[rA % nang], float_bits_abs(mA));
+            if (hasB) atomicMax(&maxabs[rB % nang], float_bits_abs(mB));
+        }
+    }
+}
+
+template <typename R, int P>
+static int launch_filter_p(const FilterLaunch &a, hipStream_t st)
+{
+    using S = FFTShape<P>;
+    long long rows = (long long)a.Nv * a.nang;
+    long long pairs = (rows + 1) / 2;
+    long long blocks = (pairs + S::NT - 1) / S::NT;
+    if (blocks == 0) return FDK_OK;
+    if (blocks > 0x7fffffffLL) return fail(FDK_ERR_UNSUPPORTED, "filter: too many rows");
+    size_t lds = sizeof(cpx<R>) * (size_t)S::NT * P;
+    auto kern = filter_kernel<R, P>;
+    if (lds > 65536) TKV_HIP(hipFuncSetAttribute((const void *)kern,
+                                                  hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                  (int)lds));
+    hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(256), lds, st, (const R *)a.in,
+                       (R *)a.out, a.Nu, a.Nv, a.nang, a.weight, (R)a.cX, (R)a.cZ, (R)a.D,
+                       (const R *)a.Hre, (const R *)a.Him, a.conj, (const cpx<R> *)a.tw,
+                       a.maxabs);
+    TKV_HIP(hipGetLastError());
+    return FDK_OK;
+}
+
+template <typename R>
+static int launch_filter_t(const FilterLaunch &a, hipStream_t st)
+{
+    switch (a.p) {
+    case 4: return launch_filter_p<R, 4>(a, st);
+    case 8: return launch_filter_p<R, 8>(a, st);
+    case 16: return launch_filter_p<R, 16>(a, st);
+    case 32: return launch_filter_p<R, 32>(a, st);
+    case 64: return launch_filter_p<R, 64>(a, st);
+    case 128: return launch_filter_p<R, 128>(a, st);
+    case 256: return launch_filter_p<R, 256>(a, st);
+    case 512: return launch_filter_p<R, 512>(a, st);
+    case 1024: return launch_filter_p<R, 1024>(a, st);
+    case 2048: return launch_filter_p<R, 2048>(a, st);
+    case 4096: return launch_filter_p<R, 4096>(a, st);
+    case 8192:
+        if (sizeof(R) == 4) return launch_filter_p<R, 8192>(a, st);
+        break;
+    default: break;
+    }
+    return fail(FDK_ERR_UNSUPPORTED,
+                "filter: padded length " + std::to_string(a.p) + " is not supported");
+}
+
+int launch_filter(int real_bytes, const FilterLaunch &a, hipStream_t st)
+{
+    return real_bytes == 8 ? launch_filter_t<double>(a, st) : launch_filter_t<float>(a, st);
+}
+
+// ---------------------------------------------------------------- tap gradient
+// For the learnable filter y = crop(IFFT_unnorm(H . FFT(pad q))), H = DFT_p(h):
+//   dL/dh[j] = p * sum_rows sum_m G[m] q[(m - j) mod p] = sum_k R[k] e^{-2 pi i k j/p},
+//   R[k] = sum_rows Q[k] conj(G^[k])          (Q, G^: DFTs of the padded rows).
+// Two rows A, B travel as one complex sequence (Zq = Q_A + i Q_B, likewise ZG);
+// then Q_A conj(G_A) + Q_B conj(G_B) = (P[k] + conj(P[-k]))/2 with
+// P[k] = Zq[k] conj(ZG[k]).  Each workgroup accumulates P over its row pairs in
+// registers and writes one partial spectrum; the host sums the partials in
+// double, symmetrises and applies the final length-p DFT.
+template <typename R, int P>
+__global__ void __launch_bounds__(256)
+    tapgrad_kernel(const R *__restrict__ q, const R *__restrict__ G, int Nu, long long rows,
+                   const cpx<R> *__restrict__ tw, cpx<R> *__restrict__ partial)
+{
+    using S = FFTShape<P>;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cpx<R> *smem = reinterpret_cast<cpx<R> *>(smem_raw);
+    const int tid = threadIdx.x;
+    const int f = tid / S::TPT;
+    const int t = tid - f * S::TPT;
+    cpx<R> *sq = smem + f * P;
+    cpx<R> *sg = smem + (S::NT + f) * P;
+    constexpr int E = P / S::TPT;
+    cpx<R> acc[E];
+#pragma unroll
+    for (int j = 0; j < E; ++j) acc[j] = {0, 0};
+    const long long pairs = (rows + 1) / 2;
+    for (long long pr0 = (long long)blockIdx.x * S::NT; pr0 < pairs;
+         pr0 += (long long)gridDim.x * S::NT) {
+        const long long pr = pr0 + f;
+        const long long rA = 2 * pr, rB = rA + 1;
+        const bool hasA = pr < pairs && rA < rows, hasB = pr < pairs && rB < rows;
+#pragma unroll
+        for (int j = 0; j < E; ++j) {
+            int n = t + j * S::TPT;
+            R qa = 0, qb = 0, ga = 0, gb = 0;
+            if (n < Nu) {
+                if (hasA) { qa = q[rA * Nu + n]; ga = G[rA * Nu + n]; }
+                if (hasB) { qb = q[rB * Nu + n]; gb = G[rB * Nu + n]; }
+            }
+            sq[n] = {qa, qb};
+            sg[n] = {ga, gb};
+        }
+        __syncthreads();
+        fft_lds<R, P, false>(sq, tw, t);
+        fft_lds<R, P, false>(sg, tw, t);
+#pragma unroll
+        for (int j = 0; j < E; ++j) {
+            int n = t + j * S::TPT;
+            cpx<R> a = sq[n], g = sg[n];
+            acc[j].x += a.x * g.x + a.y * g.y;   // a * conj(g)
+            acc[j].y += a.y * g.x - a.x * g.y;
+        }
+        __syncthreads();
+    }
+    // reduce the NT transforms of the workgroup into transform 0's layout
+    cpx<R> *red = smem;
+#pragma unroll
+    for (int j = 0; j < E; ++j) red[f * P + t + j * S::TPT] = acc[j];
+    __syncthreads();
+    for (int n = tid; n < P; n += 256) {
+        cpx<R> s = {0, 0};
+        for (int ff = 0; ff < S::NT; ++ff) {
+            s.x += red[ff * P + n].x;
+            s.y += red[ff * P + n].y;
+        }
+        partial[(size_t)blockIdx.x * P + n] = s;
+    }
+}
+
+template <typename R, int P>
+static int launch_tapgrad_p(const TapGradLaunch &a, hipStream_t st)
+{
+    using S = FFTShape<P>;
+    size_t lds = sizeof(cpx<R>) * (size_t)S::NT * P * 2;
+    auto kern = tapgrad_kernel<R, P>;
+    if (lds > 65536) TKV_HIP(hipFuncSetAttribute((const void *)kern,
+                                                  hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                  (int)lds));
+    hipLaunchKernelGGL(kern, dim3((unsigned)a.nblocks), dim3(256), lds, st, (const R *)a.q,
+                       (const R *)a.G, a.Nu, (long long)a.rows, (const cpx<R> *)a.tw,
+                       (cpx<R> *)a.partial);
+    TKV_HIP(hipGetLastError());
+    return FDK_OK;
+}
+
+template <typename R>
+static int launch_tapgrad_t(const TapGradLaunch &a, hipStream_t st)
+{
+    switch (a.p) {
+    case 4: return launch_tapgrad_p<R, 4>(a, st);
+    case 8: return launch_tapgrad_p<R, 8>(a, st);
+    case 16: return launch_tapgrad_p<R, 16>(a, st);
+    case 32: return launch_tapgrad_p<R, 32>(a, st);
+    case 64: return launch_tapgrad_p<R, 64>(a, st);
+    case 128: return launch_tapgrad_p<R, 128>(a, st);
+    case 256: return launch_tapgrad_p<R, 256>(a, st);
+    case 512: return launch_tapgrad_p<R, 512>(a, st);
+    case 1024: return launch_tapgrad_p<R, 1024>(a, st);
+    case 2048: return launch_tapgrad_p<R, 2048>(a, st);
+    case 4096:
+        if (sizeof(R) == 4) return launch_tapgrad_p<R, 4096>(a, st);
+        break;
+    default: break;
+    }
+    return fail(FDK_ERR_UNSUPPORTED,
+                "tap gradient: padded length " + std::to_string(a.p) + " is not supported");
+}
+
+int launch_tap_gradient(int real_bytes, const TapGradLaunch &a, hipStream_t st)
+{
+    return real_bytes == 8 ? launch_tapgrad_t<double>(a, st) : launch_tapgrad_t<float>(a, st);
+}
+
+// ---------------------------------------------------------------- repack to cells
+// filtered [v][beta][u] (R)  ->  cells [beta][v][pitch]:
+//   cell(beta, v, u) = (Q[v][beta][u], Q[v+1][beta][u]) / scale_beta
+// Each thread walks VB consecutive rows of one column, reading each sample once.
+template <typename Cell>
+struct CellTraits;
+template <>
+struct CellTraits<float2> {
+    static __device__ __forceinline__ float2 make(float a, float b) { return {a, b}; }
+};
+template <>
+struct CellTraits<double2> {
+    static __device__ __forceinline__ double2 make(double a, double b) { return {a, b}; }
+};
+struct half2u {
+    unsigned v;
+};
+struct bf162u {
+    unsigned v;
+};
+template <>
+struct CellTraits<half2u> {
+    static __device__ __forceinline__ half2u make(float a, float b)
+    {
+        unsigned lo = __half_as_ushort(__float2half_rn(a));
+        unsigned hi = __half_as_ushort(__float2half_rn(b));
+        return {lo | (hi << 16)};
+    }
+};
+template <>
+struct CellTraits<bf162u> {
+    static __device__ __forceinline__ bf162u make(float a, float b)
+    {
+        unsigned lo = __bfloat16_as_ushort(__float2bfloat16(a));
+        unsigned hi = __bfloat16_as_ushort(__float2bfloat16(b));
+        return {lo | (hi << 16)};
+    }
+};
+
+constexpr int PACK_VB = 16;
+// FP16 keeps the largest filtered magnitude of a projection at 2^14, far from
+// the FP16 maximum 65504 and 2^28 above its smallest normal number.
+constexpr float F16_TARGET = 16384.0f;
+
+template <typename R, typename Cell, bool SCALED>
+__global__ void __launch_bounds__(256)
+    pack_kernel(const R *__restrict__ filt, int Nu, int Nv, int nang, int pitch,
+                const unsigned *__restrict__ maxabs, Cell *__restrict__ cells,
+                R *__restrict__ scales)
+{
+    const int u = blockIdx.x * 256 + threadIdx.x;
+    const int beta = blockIdx.z;
+    const int v0 = blockIdx.y * PACK_VB;
+    if (u >= pitch) return;
+    R inv = 1, sc = 1;
+    if (SCALED) {
+        float m = __uint_as_float(maxabs[beta]);
+        sc = m > 0 ? (R)(m / F16_TARGET) : (R)1;
+        inv = (R)1 / sc;
+    }
+    if (u == 0 && blockIdx.y == 0) scales[beta] = sc;
+    const bool col = u < Nu;
+    const size_t rstride = (size_t)nang * Nu;
+    const R *src = filt + (size_t)beta * Nu + u;
+    Cell *dst = cells + ((size_t)beta * Nv) * pitch + u;
+    R prev = (col && v0 < Nv) ? src[(size_t)v0 * rstride] : (R)0;
+    const int vend = min(v0 + PACK_VB, Nv);
+    for (int v = v0; v < vend; ++v) {
+        R next = (col && v + 1 < Nv) ? src[(size_t)(v + 1) * rstride] : (R)0;
+        dst[(size_t)v * pitch] = CellTraits<Cell>::make(prev * inv, next * inv);
+        prev = next;
+    }
+}
+
+int launch_pack(int dtype_in, const void *filt, int Nu, int Nv, int nang, int pitch,
+                const unsigned *maxabs, int cell_dtype, void *cells, void *scales,
+                hipStream_t st)
+{
+    if (nang == 0 || Nv == 0) return FDK_OK;
+    if (nang > 65535) return fail(FDK_ERR_UNSUPPORTED, "pack: more than 65535 projections");
+    dim3 grid((pitch + 255) / 256, (Nv + PACK_VB - 1) / PACK_VB, nang);
+    if (dtype_in == FDK_F64) {
+        if (cell_dtype != FDK_F64)
+            return fail(FDK_ERR_INVALID_ARGUMENT, "pack: F64 input needs F64 cells");
+        hipLaunchKernelGGL((pack_kernel<double, double2, false>), grid, dim3(256), 0, st,
+                           (const double *)filt, Nu, Nv, nang, pitch, maxabs, (double2 *)cells,
+                           (double *)scales);
+    } else {
+        const float *f = (const float *)filt;
+        switch (cell_dtype) {
+        case FDK_F32:
+            hipLaunchKernelGGL((pack_kernel<float, float2, false>), grid, dim3(256), 0, st, f,
+                               Nu, Nv, nang, pitch, maxabs, (float2 *)cells, (float *)scales);
+            break;
+        case FDK_F16:
+            if (!maxabs) return fail(FDK_ERR_INVALID_ARGUMENT, "pack: FP16 needs maxabs");
+            hipLaunchKernelGGL((pack_kernel<float, half2u, true>), grid, dim3(256), 0, st, f,
+                               Nu, Nv, nang, pitch, maxabs, (half2u *)cells, (float *)scales);
+            break;
+        case FDK_BF16:
+            hipLaunchKernelGGL((pack_kernel<float, bf162u, false>), grid, dim3(256), 0, st, f,
+                               Nu, Nv, nang, pitch, maxabs, (bf162u *)cells, (float *)scales);
+            break;
+        default:
+            return fail(FDK_ERR_INVALID_ARGUMENT, "pack: F32 input needs F32/F16/BF16 cells");
+        }
+    }
+    TKV_HIP(hipGetLastError());
+    return FDK_OK;
+}
+
+// ---------------------------------------------------------------- accumulate
+// CollectionKernel (kernel.cu:203-215): slab += partial, flat and coalesced.
+__global__ void __launch_bounds__(256) accumulate_kernel(float *__restrict__ dst,
+                                                         const float *__restrict__ src,
+                                                         size_t n)
+{
+    size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * 256;
+    for (; i < n; i += stride) dst[i] += src[i];
+}
+
+int launch_accumulate(float *dst, const float *src, size_t n, hipStream_t st)
+{
+    if (n == 0) return FDK_OK;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(accumulate_kernel, dim3((unsigned)blocks), dim3(256), 0, st, dst, src,
+                       n);
+    TKV_HIP(hipGetLastError());
+    return FDK_OK;
+}
+
+// zero fill by a kernel (no memset node, so the sequence stays graph-capturable)
+__global__ void __launch_bounds__(256) zero_u32_kernel(unsigned *__restrict__ dst, size_t n)
+{
+    size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) dst[i] = 0u;
+}
+
+int launch_zero_u32(unsigned *dst, size_t n, hipStream_t st)
+{
+    if (n == 0) return FDK_OK;
+    hipLaunchKernelGGL(zero_u32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, dst,
+                       n);
+    TKV_HIP(hipGetLastError());
+    return FDK_OK;
+}
+
+// ---------------------------------------------------------------- compare
+// per-block partial sums in double, combined with double atomics:
+// out[0] += sum (a-b)^2, out[1] += sum b^2, out[2] = max |a-b| (as ordered bits)
+__global__ void __launch_bounds__(256) compare_kernel(const float *__restrict__ a,
+                                                      const float *__restrict__ b, size_t n,
+                                                      double *__restrict__ out)
+{
+    __shared__ double s0[256], s1[256], s2[256];
+    double e = 0, r = 0, m = 0;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        double d = (double)a[i] - (double)b[i];
+        e += d * d;
+        r += (double)b[i] * (double)b[i];
+        double ad = d < 0 ? -d : d;
+        m = ad > m ? ad : m;
+    }
+    s0[threadIdx.x] = e;
+    s1[threadIdx.x] = r;
+    s2[threadIdx.x] = m;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) {
+            s0[threadIdx.x] += s0[threadIdx.x + off];
+            s1[threadIdx.x] += s1[threadIdx.x + off];
+            s2[threadIdx.x] = s2[threadIdx.x] > s2[threadIdx.x + off] ? s2[threadIdx.x]
+                                                                     : s2[threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        unsafeAtomicAdd(&out[0], s0[0]);
+        unsafeAtomicAdd(&out[1], s1[0]);
+        atomicMax((unsigned long long *)&out[2], (unsigned long long)__double_as_longlong(s2[0]));
+    }
+}
+
+__global__ void zero_f64x3(double *out)
+{
+    if (threadIdx.x < 3) out[threadIdx.x] = 0.0;
+}
+
+int launch_compare(const float *a, const float *b, size_t n, double *d_out3, hipStream_t st)
+{
+    hipLaunchKernelGGL(zero_f64x3, dim3(1), dim3(64), 0, st, d_out3);
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    if (blocks == 0) blocks = 1;
+    hipLaunchKernelGGL(compare_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a, b, n, d_out3);
+    TKV_HIP(hipGetLastError());
+    return FDK_OK;
+}
+
+}  // namespace tkv

@@ -1,0 +1,132 @@
+// fdk_internal.hpp -- shared declarations of the MI355X FDK library (not installed).
+//
+// Device data layouts (all in HBM, one allocation each; 288 GB holds every
+// configuration without tiling):
+//   input stack   float [v][beta][u]          as the host hands it over
+//                 (sinogram-major, FBPG/mucat_file.cpp:244-267)
+//   cell stack    Cell  [beta][v][pitch]      filtered projections, angle-major;
+//                 cell(beta,v,u) = (Q[beta][v][u], Q[beta][v+1][u]) so one
+//                 load of two adjacent cells returns the whole 2x2 bilinear
+//                 neighbourhood; row Nv and columns >= Nu are zero.
+//   volume        float [z][y][x]             the full volume, slabs are z-ranges
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "tomokvas/fdk.h"
+
+namespace tkv {
+
+// ---------------------------------------------------------------- errors
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+int hip_fail(hipError_t e, const char *what);
+
+#define TKV_HIP(call)                                            \
+    do {                                                         \
+        hipError_t e_ = (call);                                  \
+        if (e_ != hipSuccess) return ::tkv::hip_fail(e_, #call); \
+    } while (0)
+
+// ---------------------------------------------------------------- geometry
+// Float-valued parameters exactly as the reference engine's setters receive them
+// (FBPG/kernel.cuh:89-101).
+struct Geometry {
+    int Nu = 0, Nang = 0, Nv = 0;
+    float cX = 0, cZ = 0, dbeta = 0, D = 0, tau = 0;
+    int Nx = 0, Ny = 0, Nz = 0;
+    float cx = 0, cy = 0, cz = 0, rot = 0, scale = 1;
+    int flipX = 0, flipY = 0, flipZ = 0;
+    int B = 16;
+};
+
+int padded_length(int Nu);
+// FP64 angle tables: angle_k = k*dbeta + rot (kernel.cu:430-441)
+void angle_tables(const Geometry &g, std::vector<double> &s, std::vector<double> &c);
+// float voxel coordinates as FBPConeKernel forms them (kernel.cu:82-88, 398-400,
+// 821); az is extended linearly to nz_pad >= Nz entries
+void axis_tables_f32(const Geometry &g, int nz_pad, std::vector<float> &ax,
+                     std::vector<float> &ay, std::vector<float> &az);
+void axis_tables_f64(const Geometry &g, int nz_pad, std::vector<double> &ax,
+                     std::vector<double> &ay, std::vector<double> &az);
+// reference slab count ceil((Nz-1)/B) and the covering count ceil(Nz/B)
+int slab_count_reference(int Nz, int B);
+int slab_count(int Nz, int B);
+
+// filter spectrum in FP64 (kernel.cu:570-655, 783-802):
+//   kind RAMLAK / SHEPPLOGAN: H[k] = tau*|DFT_p(t)[k]| real, k in [0,p)
+//   kind USER: H[k] = DFT_p(h)[k] complex for the caller's p spatial taps
+void build_spectrum(int kind, const Geometry &g, const std::vector<double> &user_taps,
+                    std::vector<double> &Hre, std::vector<double> &Him);
+
+// ---------------------------------------------------------------- kernel launchers
+struct FilterLaunch {
+    const void *in;         // [rows][Nu] rows = Nv * nang, Real
+    void *out;              // may alias in
+    int Nu, Nv, nang, p;
+    int weight;             // fuse the cosine pre-weight into the load
+    float cX, cZ, D;
+    const void *Hre, *Him;  // device spectrum [p] (Real), Him may be null (real H)
+    int conj;               // multiply by conj(H) (adjoint)
+    const void *tw;         // device twiddles exp(-2 pi i j/p), [p] complex Real
+    unsigned *maxabs;       // per local angle, may be null (float bits)
+};
+int launch_weight(int real_bytes, void *sino, int Nu, int Nv, int nang, float cX, float cZ,
+                  float D, hipStream_t st);
+int launch_filter(int real_bytes, const FilterLaunch &a, hipStream_t st);
+int launch_pack(int dtype_in, const void *filt, int Nu, int Nv, int nang, int pitch,
+                const unsigned *maxabs, int cell_dtype, void *cells, void *scales,
+                hipStream_t st);
+int launch_accumulate(float *dst, const float *src, size_t n, hipStream_t st);
+int launch_zero_u32(unsigned *dst, size_t n, hipStream_t st);
+int launch_compare(const float *a, const float *b, size_t n, double *d_out3, hipStream_t st);
+
+struct BPLaunch {
+    const void *cells;      // [nang][Nv][pitch]
+    const void *scales;     // per local angle dequantisation factor (Real)
+    const void *ang;        // per local angle (cos, sin) pairs (Real)
+    const void *ax, *ay, *az;  // coordinate tables (Real), az padded to the z-run
+    void *vol;              // [Nz][Ny][Nx] Real
+    int Nu, Nv, nang, pitch;
+    int Nx, Ny, Nz;
+    double cX, cZ, D, out_scale, maskR2;
+    int accumulate;
+};
+int launch_backproject(int cell_dtype, const BPLaunch &a, hipStream_t st);
+int bp_zrun(int cell_dtype);
+
+struct FPLaunch {           // adjoint of the backprojector
+    const void *vol;        // [Nz][Ny][Nx]
+    void *sino;             // [Nv][nang][Nu], accumulated into (caller clears)
+    const void *ang, *ax, *ay, *az;
+    int Nu, Nv, nang;
+    int Nx, Ny, Nz;
+    double cX, cZ, D, out_scale, maskR2;
+};
+int launch_forward_project(int real_bytes, const FPLaunch &a, hipStream_t st);
+
+struct TapGradLaunch {
+    const void *q, *G;      // [rows][Nu] weighted input rows and output gradients
+    int Nu, rows, p;
+    const void *tw;
+    void *partial;          // [nblocks][p] complex Real
+    int nblocks;
+};
+int launch_tap_gradient(int real_bytes, const TapGradLaunch &a, hipStream_t st);
+
+struct PhantomLaunch {
+    const double *ells;     // [n][19]
+    int n;
+    const double *ang;      // (cos, sin) per local angle
+    float *sino;            // [Nv][nang][Nu]
+    int Nu, Nv, nang, ang0_stride;
+    double cX, cZ, D, tau;
+};
+int launch_phantom(const PhantomLaunch &a, hipStream_t st);
+int launch_poisson_log(float *p, size_t n, double I0, uint64_t seed, hipStream_t st);
+
+}  // namespace tkv
