@@ -6,7 +6,7 @@
  * it.  Every engine call below names the reference member it replaces.  Plain C
  * types only: host buffers are borrowed for the duration of a call, device
  * buffers of the engine are owned by the context, streams are passed as
- * `void*` (a hipStream_t, NULL = the context's stream).
+ * `void*` (a hipStream_t; NULL = the default stream).
  *
  * Return codes: 0 = success (the reference returns cudaSuccess), otherwise one
  * of the FDK_ERR_* values; fdk_last_error_string() describes the last failure

@@ -6,9 +6,20 @@
  *   REAL = float,  SUF(x) = x##_f32, OFDK_F64 = 0   (FP32 mirror)
  *   REAL = double, SUF(x) = x##_f64, OFDK_F64 = 1   (FP64 oracle)
  * Compiled with -ffp-contract=off so every float expression is evaluated
- * exactly as written (one rounding per operation), like the reference's
- * separate float operations.
+ * exactly as written.  Where nvcc's default contraction (-fmad=true) turns a
+ * product feeding an addition of the reference's device code into one fused
+ * multiply-add, this restatement writes that fma explicitly (SUF(ffma)); all
+ * other operations are single roundings in the reference's order.
  */
+
+static inline REAL SUF(ffma)(REAL a, REAL b, REAL c)
+{
+#if OFDK_F64
+    return fma(a, b, c);
+#else
+    return fmaf(a, b, c);
+#endif
+}
 
 /* ---- voxel coordinates ------------------------------------------------------- */
 /* FP32 mirror: the float values the reference kernel forms (kernel.cu:82-88,
@@ -56,7 +67,9 @@ void SUF(ofdk_weight)(const ofdk_geom *g, REAL *sino)
             REAL *row = sino + ((size_t)v * g->Nang + b) * g->Nu;
             for (int u = 0; u < g->Nu; ++u) {
                 REAL ce = (REAL)u - (REAL)g->cX;
-                row[u] *= D / (REAL)sqrt((double)(ce * ce + D * D + cs * cs));
+                /* ce*ce + D*D + cs*cs, contracted left to right */
+                REAL t = SUF(ffma)(cs, cs, SUF(ffma)(ce, ce, D * D));
+                row[u] *= D / (REAL)sqrt((double)t);
             }
         }
     }
@@ -182,10 +195,11 @@ void SUF(ofdk_filter_complex)(const ofdk_geom *g, const double *Hre,
  * closed detector rectangle [0,Nu-1] x [0,Nv-1]; ceil/floor formulation of the
  * reference (interpolation from the upper sample towards the lower one). */
 static inline REAL SUF(fq3d)(const ofdk_geom *g, const REAL *sino, int beta,
-                             REAL pp, REAL ff)
+                             REAL T, REAL U, REAL aZ)
 {
-    REAL scP = pp + (REAL)g->cX;
-    REAL scF = ff + (REAL)g->cZ;
+    /* fQ3D(T*U, aZ*U, ...) then _p + cX, _f + cZ: contracted to fmas */
+    REAL scP = SUF(ffma)(T, U, (REAL)g->cX);
+    REAL scF = SUF(ffma)(aZ, U, (REAL)g->cZ);
     if (scP < 0 || scP > (REAL)(g->Nu - 1)) return 0;
     if (scF < 0 || scF > (REAL)(g->Nv - 1)) return 0;
     int af = (int)ceil(scF), bf = (int)floor(scF);
@@ -195,15 +209,15 @@ static inline REAL SUF(fq3d)(const ofdk_geom *g, const REAL *sino, int beta,
     if (af != bf) {
         REAL va, vb;
         if (a != b) {
-            va = rowa[a] + (rowa[b] - rowa[a]) * (-scP + (REAL)a);
-            vb = rowb[a] + (rowb[b] - rowb[a]) * (-scP + (REAL)a);
+            va = SUF(ffma)(rowa[b] - rowa[a], (REAL)a - scP, rowa[a]);
+            vb = SUF(ffma)(rowb[b] - rowb[a], (REAL)a - scP, rowb[a]);
         } else {
             va = rowa[a];
             vb = rowb[b];
         }
-        return va + (vb - va) * (-scF + (REAL)af);
+        return SUF(ffma)(vb - va, (REAL)af - scF, va);
     }
-    if (a != b) return rowa[a] + (rowa[b] - rowa[a]) * (-scP + (REAL)a);
+    if (a != b) return SUF(ffma)(rowa[b] - rowa[a], (REAL)a - scP, rowa[a]);
     return rowa[a];
 }
 
@@ -239,10 +253,10 @@ void SUF(ofdk_backproject)(const ofdk_geom *g, const REAL *sino, REAL *vol)
                     for (int x = 0; x < g->Nx; ++x) {
                         const REAL aX = ax[x];
                         if (!SUF(inside)(g, aX, aY)) continue;
-                        REAL S = aX * c + aY * s;
-                        REAL T = aY * c - aX * s;
+                        REAL S = SUF(ffma)(aX, c, aY * s);
+                        REAL T = SUF(ffma)(aY, c, -(aX * s));
                         REAL U = D / (D - S);
-                        acc[x] += U * U * SUF(fq3d)(g, sino, b, T * U, aZ * U);
+                        acc[x] = SUF(ffma)(U * U, SUF(fq3d)(g, sino, b, T, U, aZ), acc[x]);
                     }
                 }
                 REAL *out = vol + ((size_t)z * g->Ny + y) * g->Nx;
@@ -278,10 +292,10 @@ void SUF(ofdk_backproject_points)(const ofdk_geom *g, const REAL *sino, int npts
         REAL acc = 0;
         for (int b = 0; b < g->Nang; ++b) {
             const REAL s = (REAL)sb[b], c = (REAL)cb[b];
-            REAL S = aX * c + aY * s;
-            REAL T = aY * c - aX * s;
+            REAL S = SUF(ffma)(aX, c, aY * s);
+            REAL T = SUF(ffma)(aY, c, -(aX * s));
             REAL U = D / (D - S);
-            acc += U * U * SUF(fq3d)(g, sino, b, T * U, aZ * U);
+            acc = SUF(ffma)(U * U, SUF(fq3d)(g, sino, b, T, U, aZ), acc);
         }
         out[i] = acc * fscale;
     }
@@ -317,11 +331,11 @@ void SUF(ofdk_forward_project)(const ofdk_geom *g, const REAL *vol, REAL *sino)
             if (!SUF(inside)(g, aX, aY)) continue;
             REAL val = vol[((size_t)z * g->Ny + y) * g->Nx + x];
             if (val == 0) continue;
-            REAL S = aX * c + aY * s;
-            REAL T = aY * c - aX * s;
+            REAL S = SUF(ffma)(aX, c, aY * s);
+            REAL T = SUF(ffma)(aY, c, -(aX * s));
             REAL U = D / (D - S);
-            REAL scP = T * U + (REAL)g->cX;
-            REAL scF = aZ * U + (REAL)g->cZ;
+            REAL scP = SUF(ffma)(T, U, (REAL)g->cX);
+            REAL scF = SUF(ffma)(aZ, U, (REAL)g->cZ);
             if (scP < 0 || scP > (REAL)(g->Nu - 1)) continue;
             if (scF < 0 || scF > (REAL)(g->Nv - 1)) continue;
             int af = (int)ceil(scF), bf = (int)floor(scF);

@@ -24,7 +24,7 @@ FDK_ERR_UNSUPPORTED = 6
 F32, F16, BF16, F64 = 0, 1, 2, 3
 FILTER_RAMLAK, FILTER_SHEPPLOGAN, FILTER_USER = 0, 1, 2
 
-GET_BLOCK_FN = C.CFUNCTYPE(C.POINTER(C.c_float), C.c_void_p, C.c_int, C.c_int)
+GET_BLOCK_FN = C.CFUNCTYPE(C.c_void_p, C.c_void_p, C.c_int, C.c_int)
 
 _vp = C.c_void_p
 _i = C.c_int

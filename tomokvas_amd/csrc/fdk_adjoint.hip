@@ -33,6 +33,13 @@ template <>
 __device__ __forceinline__ double rfl(double x) { return floor(x); }
 
 template <typename R>
+__device__ __forceinline__ R rfm(R a, R b, R c);
+template <>
+__device__ __forceinline__ float rfm(float a, float b, float c) { return fmaf(a, b, c); }
+template <>
+__device__ __forceinline__ double rfm(double a, double b, double c) { return fma(a, b, c); }
+
+template <typename R>
 __device__ __forceinline__ void splat(R *p, R v)
 {
     unsafeAtomicAdd(p, v);
@@ -68,10 +75,10 @@ __global__ void __launch_bounds__(256) fp_kernel(FPArgs<R> a)
     const size_t rstride = (size_t)a.nang * a.Nu;   // one detector row to the next
     for (int b = 0; b < a.nang; ++b) {
         const R c = a.ang[2 * b], s = a.ang[2 * b + 1];
-        R S = aX * c + aY * s;
-        R T = aY * c - aX * s;
+        R S = rfm(aX, c, aY * s);
+        R T = rfm(aY, c, -(aX * s));
         R U = a.D / (a.D - S);
-        R u = T * U + a.cX;
+        R u = rfm(T, U, a.cX);
         if (!(u >= (R)0 && u <= a.umax)) continue;
         R fl = rfl(u);
         const int iu = (int)fl;
@@ -83,7 +90,7 @@ __global__ void __launch_bounds__(256) fp_kernel(FPArgs<R> a)
 #pragma unroll
         for (int k = 0; k < ZR; ++k) {
             if (val[k] == (R)0) continue;
-            R v = az[k] * U + a.cZ;
+            R v = rfm(az[k], U, a.cZ);
             if (!(v >= (R)0 && v <= a.vmax)) continue;
             R fv0 = rfl(v);
             const int iv = (int)fv0;

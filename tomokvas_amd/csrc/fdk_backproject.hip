@@ -7,24 +7,44 @@
 //   samples outside [0,Nu-1] x [0,Nv-1] contribute 0.
 //
 // MI355X design:
-//  * a thread owns one (x,y) column and a run of ZR consecutive slices kept in
-//    registers: S, T, U and the detector column u = T*U + cX depend only on
-//    (x, y, beta), so the per-angle geometry is paid once per ZR updates;
-//  * per update: v = aZ*U + cZ, floor/fract, one address, ONE load of two
-//    adjacent cells = the whole 2x2 bilinear neighbourhood, 6 FMA-class ops;
-//  * a wave covers an 8x8 column tile (a compact detector footprint per
-//    angle), a workgroup 16x16 columns; all lanes walk the angles in lockstep,
-//    so the per-angle cos/sin/scale and the base address are scalar (SGPR);
+//  * filtered projections are stored as "quad cells": for every detector
+//    sample (v,u) the four bilinear coefficients
+//        A = Q(v,u)              B = Q(v+1,u) - Q(v,u)
+//        C = Q(v,u+1) - Q(v,u)   D = Q(v+1,u+1) - Q(v+1,u) - Q(v,u+1) + Q(v,u)
+//    so bilinear(v+fv, u+fu) = (A + fu C) + fv (B + fu D): ONE aligned 16-byte
+//    (FP32) or 8-byte (FP16) load and three FMAs per voxel-update;
+//  * a thread owns one (x,y) column and a run of ZR consecutive slices in
+//    registers: S, T, U, u = T*U + cX, fu and U^2 depend only on (x, y, beta)
+//    and are paid once per ZR updates;
+//  * per update: v = aZ*U + cZ, row = trunc(v) (v >= 0 there), fv = fract(v),
+//    one address, one load, 4 FMAs;
+//  * a wave covers an 8x8 column tile (compact detector footprint), a
+//    workgroup 16x16 columns; all lanes walk the angles in lockstep, so the
+//    per-angle cos/sin/scale and the projection base address are scalar;
 //  * z-runs that project entirely inside the detector for every lane of the
 //    wave (the common case) skip the per-slice range test;
 //  * workgroup ids are remapped so the workgroups an XCD runs together cover a
 //    compact block of the volume and share detector rows in that XCD's L2;
-//  * FP16 cells feed v_fma_mix_f32 directly: FP16 taps, FP32 weights and FP32
-//    accumulators, half the gather bytes of FP32 cells.
+//  * FP16 quads feed v_fma_mix_f32 directly: FP16 coefficients, FP32 weights
+//    and FP32 accumulators, half the gather bytes of FP32 quads.
 #include "fdk_internal.hpp"
 
 #include <hip/hip_bf16.h>
 #include <hip/hip_fp16.h>
+
+// Tuning knobs (defaults chosen by measurement on MI355X):
+//   TKV_BP_ZR            slices per thread (register z-run) for FP32/FP16/BF16
+//   TKV_BP_LB            gathers per lane issued together
+//   TKV_BP_BATCH_BARRIER 1 = no load is scheduled ahead of the previous batch
+#ifndef TKV_BP_ZR
+#define TKV_BP_ZR 32
+#endif
+#ifndef TKV_BP_LB
+#define TKV_BP_LB 8
+#endif
+#ifndef TKV_BP_BATCH_BARRIER
+#define TKV_BP_BATCH_BARRIER 1
+#endif
 
 namespace tkv {
 
@@ -44,100 +64,69 @@ struct BPArgs {
     int tiles_x, tiles_xy, nzrun, nblocks;
 };
 
-// ---- tap access ------------------------------------------------------------------
-// load(): the two adjacent cells at byte offset `off` (the 2x2 neighbourhood);
-// lerp(): c0 = (1-fv) Q(v,u) + fv Q(v+1,u), c1 likewise at u+1.  Loads of a whole
-// z-run are issued before any is consumed, so each wave keeps ZR gathers in
-// flight.
-template <typename Cell>
-struct Taps;
+// ---- quad cells: load one, evaluate (A + fu C) + fv (B + fu D) ------------------------
+struct quad_f32;
+struct quad_f16;
+struct quad_bf16;
+struct quad_f64;
+
+template <typename Q>
+struct Quad;
 
 template <>
-struct Taps<float2> {
+struct Quad<quad_f32> {
     using R = float;
-    using L = float4;
-    static constexpr int BYTES = 8;
-    static __device__ __forceinline__ L load(const char *base, uint32_t off)
+    using L = float4;   // (A, B, C, D)
+    static constexpr int BYTES = 16;
+    static __device__ __forceinline__ float eval(const L &q, float fu, float fv)
     {
-        L q;
-        __builtin_memcpy(&q, base + off, sizeof(L));
-        return q;
-    }
-    static __device__ __forceinline__ void lerp(const L &q, float fv, float gv, float &c0,
-                                                float &c1)
-    {
-        c0 = fmaf(fv, q.y - q.x, q.x);
-        c1 = fmaf(fv, q.w - q.z, q.z);
+        float t0 = fmaf(fu, q.z, q.x);
+        float t1 = fmaf(fu, q.w, q.y);
+        return fmaf(fv, t1, t0);
     }
 };
 
 template <>
-struct Taps<double2> {
+struct Quad<quad_f64> {
     using R = double;
     using L = double4;
-    static constexpr int BYTES = 16;
-    static __device__ __forceinline__ L load(const char *base, uint32_t off)
+    static constexpr int BYTES = 32;
+    static __device__ __forceinline__ double eval(const L &q, double fu, double fv)
     {
-        L q;
-        __builtin_memcpy(&q, base + off, sizeof(L));
-        return q;
-    }
-    static __device__ __forceinline__ void lerp(const L &q, double fv, double gv, double &c0,
-                                                double &c1)
-    {
-        c0 = fma(fv, q.y - q.x, q.x);
-        c1 = fma(fv, q.w - q.z, q.z);
-    }
-};
-
-struct half2u {
-    unsigned v;
-};
-struct bf162u {
-    unsigned v;
-};
-
-template <>
-struct Taps<half2u> {
-    using R = float;
-    using L = uint2;
-    static constexpr int BYTES = 4;
-    static __device__ __forceinline__ L load(const char *base, uint32_t off)
-    {
-        L q;
-        __builtin_memcpy(&q, base + off, sizeof(L));
-        return q;
-    }
-    // every FP16 operand sits in an FMA with FP32 weights: v_fma_mix_f32,
-    // no separate conversions
-    static __device__ __forceinline__ void lerp(const L &q, float fv, float gv, float &c0,
-                                                float &c1)
-    {
-        __half2 h0 = *reinterpret_cast<const __half2 *>(&q.x);
-        __half2 h1 = *reinterpret_cast<const __half2 *>(&q.y);
-        c0 = fmaf(fv, __high2float(h0), fmaf(gv, __low2float(h0), 0.0f));
-        c1 = fmaf(fv, __high2float(h1), fmaf(gv, __low2float(h1), 0.0f));
+        double t0 = fma(fu, q.z, q.x);
+        double t1 = fma(fu, q.w, q.y);
+        return fma(fv, t1, t0);
     }
 };
 
 template <>
-struct Taps<bf162u> {
+struct Quad<quad_f16> {
+    using R = float;
+    using L = uint2;    // x = (A, B) halves, y = (C, D) halves
+    static constexpr int BYTES = 8;
+    // FP16 operands fold into v_fma_mix_f32: FP32 arithmetic, no conversions
+    static __device__ __forceinline__ float eval(const L &q, float fu, float fv)
+    {
+        __half2 ab = *reinterpret_cast<const __half2 *>(&q.x);
+        __half2 cd = *reinterpret_cast<const __half2 *>(&q.y);
+        float t0 = fmaf(fu, __low2float(cd), __low2float(ab));
+        float t1 = fmaf(fu, __high2float(cd), __high2float(ab));
+        return fmaf(fv, t1, t0);
+    }
+};
+
+template <>
+struct Quad<quad_bf16> {
     using R = float;
     using L = uint2;
-    static constexpr int BYTES = 4;
-    static __device__ __forceinline__ L load(const char *base, uint32_t off)
+    static constexpr int BYTES = 8;
+    static __device__ __forceinline__ float eval(const L &q, float fu, float fv)
     {
-        L q;
-        __builtin_memcpy(&q, base + off, sizeof(L));
-        return q;
-    }
-    static __device__ __forceinline__ void lerp(const L &q, float fv, float gv, float &c0,
-                                                float &c1)
-    {
-        float a00 = __uint_as_float(q.x << 16), a10 = __uint_as_float(q.x & 0xffff0000u);
-        float a01 = __uint_as_float(q.y << 16), a11 = __uint_as_float(q.y & 0xffff0000u);
-        c0 = fmaf(fv, a10 - a00, a00);
-        c1 = fmaf(fv, a11 - a01, a01);
+        float A = __uint_as_float(q.x << 16), B = __uint_as_float(q.x & 0xffff0000u);
+        float C = __uint_as_float(q.y << 16), D = __uint_as_float(q.y & 0xffff0000u);
+        float t0 = fmaf(fu, C, A);
+        float t1 = fmaf(fu, D, B);
+        return fmaf(fv, t1, t0);
     }
 };
 
@@ -153,13 +142,26 @@ template <>
 __device__ __forceinline__ float rfma(float a, float b, float c) { return fmaf(a, b, c); }
 template <>
 __device__ __forceinline__ double rfma(double a, double b, double c) { return fma(a, b, c); }
+// x - floor(x) in one instruction (v_fract); exact for the finite x used here
+template <typename R>
+__device__ __forceinline__ R rfract(R x);
+template <>
+__device__ __forceinline__ float rfract(float x) { return __builtin_amdgcn_fractf(x); }
+template <>
+__device__ __forceinline__ double rfract(double x) { return __builtin_amdgcn_fract(x); }
 
-template <typename Cell, int ZR, int LB>
-__global__ void __launch_bounds__(256)
-    bp_kernel(BPArgs<typename Taps<Cell>::R> a)
+template <typename Q, typename L>
+__device__ __forceinline__ L load_quad(const char *base, uint32_t off)
 {
-    using R = typename Taps<Cell>::R;
-    using TP = Taps<Cell>;
+    return *reinterpret_cast<const L *>(base + off);
+}
+
+template <typename Q, int ZR, int LB>
+__global__ void __launch_bounds__(256) bp_kernel(BPArgs<typename Quad<Q>::R> a)
+{
+    using R = typename Quad<Q>::R;
+    using QT = Quad<Q>;
+    using L = typename QT::L;
 
     // XCD-aware block order: hardware dispatches workgroup i to XCD i % 8; the
     // remap gives each XCD a contiguous range of logical blocks.
@@ -197,50 +199,42 @@ __global__ void __launch_bounds__(256)
         for (int b = 0; b < a.nang; ++b, proj += a.angle_bytes) {
             const R c = a.ang[2 * b], s = a.ang[2 * b + 1];
             const R sc = a.scales[b];
-            R S = aX * c + aY * s;
-            R T = aY * c - aX * s;
+            // the reference's S, T, U and u with nvcc's contractions
+            R S = rfma(aX, c, aY * s);
+            R T = rfma(aY, c, -(aX * s));
             R U = a.D / (a.D - S);
-            R u = T * U + a.cX;
+            R u = rfma(T, U, a.cX);
             const bool act = inside && u >= (R)0 && u <= a.umax;
-            R fl = rfloor(u);
-            int iu = act ? (int)fl : 0;
-            R fu = u - fl;
-            R W = U * U * sc;
-            R wu1 = act ? W * fu : (R)0;
-            R wu0 = act ? W - W * fu : (R)0;
+            const R fu = rfract(u);
+            const int iu = act ? (int)u : 0;            // u >= 0: trunc = floor
+            const R W = act ? U * U * sc : (R)0;
             const R Uv = act ? U : (R)0;
             const R cz = act ? a.cZ : (R)0;
-            const uint32_t colb = (uint32_t)iu * (uint32_t)TP::BYTES;
+            const uint32_t colb = (uint32_t)iu * (uint32_t)QT::BYTES;
             const R v_lo = rfma(az[0], Uv, cz), v_hi = rfma(az[ZR - 1], Uv, cz);
             const R vmin = v_lo < v_hi ? v_lo : v_hi, vmax = v_lo < v_hi ? v_hi : v_lo;
             // (a 0.01-row margin keeps float rounding of the slice table away
             // from the unchecked path's row bounds)
-            if (__all(vmin >= (R)0.01 && vmax <= a.vmax - (R)0.01)) {
-                // whole run inside the detector for every lane: no range tests
+            const bool fast = __all(vmin >= (R)0.01 && vmax <= a.vmax - (R)0.01);
 #pragma unroll
-                for (int k0 = 0; k0 < ZR; k0 += LB) {
+            for (int k0 = 0; k0 < ZR; k0 += LB) {
+                R val[LB];
+                if (fast) {
+                    // whole run inside the detector for every lane: no range tests
                     uint32_t off[LB];
                     R fv[LB];
 #pragma unroll
                     for (int k = 0; k < LB; ++k) {
                         R v = rfma(az[k0 + k], Uv, cz);
-                        R fl = rfloor(v);
-                        fv[k] = v - fl;
-                        off[k] = __umul24((uint32_t)(int)fl, a.row_bytes) + colb;
+                        fv[k] = rfract(v);
+                        off[k] = __umul24((uint32_t)(int)v, a.row_bytes) + colb;
                     }
-                    typename TP::L q[LB];
+                    L q[LB];
 #pragma unroll
-                    for (int k = 0; k < LB; ++k) q[k] = TP::load(proj, off[k]);
+                    for (int k = 0; k < LB; ++k) q[k] = load_quad<Q, L>(proj, off[k]);
 #pragma unroll
-                    for (int k = 0; k < LB; ++k) {
-                        R c0, c1;
-                        TP::lerp(q[k], fv[k], (R)1 - fv[k], c0, c1);
-                        acc[k0 + k] = rfma(wu1, c1, rfma(wu0, c0, acc[k0 + k]));
-                    }
-                }
-            } else {
-#pragma unroll
-                for (int k0 = 0; k0 < ZR; k0 += LB) {
+                    for (int k = 0; k < LB; ++k) val[k] = QT::eval(q[k], fu, fv[k]);
+                } else {
                     uint32_t off[LB];
                     R fv[LB];
                     bool ok[LB];
@@ -248,21 +242,23 @@ __global__ void __launch_bounds__(256)
                     for (int k = 0; k < LB; ++k) {
                         R v = rfma(az[k0 + k], Uv, cz);
                         ok[k] = v >= (R)0 && v <= a.vmax;
-                        R fl = rfloor(v);
-                        fv[k] = v - fl;
-                        off[k] = __umul24(ok[k] ? (uint32_t)(int)fl : 0u, a.row_bytes) + colb;
+                        fv[k] = rfract(v);
+                        off[k] = __umul24(ok[k] ? (uint32_t)(int)v : 0u, a.row_bytes) + colb;
                     }
-                    typename TP::L q[LB];
+                    L q[LB];
 #pragma unroll
-                    for (int k = 0; k < LB; ++k) q[k] = TP::load(proj, off[k]);
+                    for (int k = 0; k < LB; ++k) q[k] = load_quad<Q, L>(proj, off[k]);
 #pragma unroll
-                    for (int k = 0; k < LB; ++k) {
-                        R c0, c1;
-                        TP::lerp(q[k], fv[k], (R)1 - fv[k], c0, c1);
-                        R w0 = ok[k] ? wu0 : (R)0, w1 = ok[k] ? wu1 : (R)0;
-                        acc[k0 + k] = rfma(w1, c1, rfma(w0, c0, acc[k0 + k]));
-                    }
+                    for (int k = 0; k < LB; ++k)
+                        val[k] = ok[k] ? QT::eval(q[k], fu, fv[k]) : (R)0;
                 }
+#pragma unroll
+                for (int k = 0; k < LB; ++k) acc[k0 + k] = rfma(W, val[k], acc[k0 + k]);
+#if TKV_BP_BATCH_BARRIER
+                // keep the next batch's loads behind this batch's arithmetic so
+                // at most LB gathers per lane are in flight (bounds the VGPRs)
+                __builtin_amdgcn_sched_barrier(0);
+#endif
             }
         }
     }
@@ -283,32 +279,43 @@ __global__ void __launch_bounds__(256)
     }
 }
 
-template <typename Cell>
+template <typename Q>
 constexpr int zrun_of()
 {
-    return sizeof(typename Taps<Cell>::R) == 8 ? 8 : 16;
+    return sizeof(typename Quad<Q>::R) == 8 ? 8 : TKV_BP_ZR;
 }
 
 int bp_zrun(int cell_dtype)
 {
     switch (cell_dtype) {
-    case FDK_F64: return zrun_of<double2>();
-    case FDK_F16: return zrun_of<half2u>();
-    case FDK_BF16: return zrun_of<bf162u>();
-    default: return zrun_of<float2>();
+    case FDK_F64: return zrun_of<quad_f64>();
+    case FDK_F16: return zrun_of<quad_f16>();
+    case FDK_BF16: return zrun_of<quad_bf16>();
+    default: return zrun_of<quad_f32>();
     }
 }
 
-template <typename Cell>
+int cell_bytes_of(int cell_dtype)
+{
+    switch (cell_dtype) {
+    case FDK_F32: return Quad<quad_f32>::BYTES;
+    case FDK_F16: return Quad<quad_f16>::BYTES;
+    case FDK_BF16: return Quad<quad_bf16>::BYTES;
+    case FDK_F64: return Quad<quad_f64>::BYTES;
+    default: return 0;
+    }
+}
+
+template <typename Q>
 static int launch_bp_t(const BPLaunch &L, hipStream_t st)
 {
-    using R = typename Taps<Cell>::R;
-    constexpr int ZR = zrun_of<Cell>();
+    using R = typename Quad<Q>::R;
+    constexpr int ZR = zrun_of<Q>();
     BPArgs<R> a;
     a.cells = (const char *)L.cells;
-    a.row_bytes = (uint32_t)((size_t)L.pitch * sizeof(Cell));
+    a.row_bytes = (uint32_t)((size_t)L.pitch * Quad<Q>::BYTES);
     a.angle_bytes = (size_t)a.row_bytes * L.Nv;
-    if ((size_t)a.row_bytes * L.Nv > 0xffffffffull)
+    if (a.angle_bytes > 0xffffffffull || a.row_bytes >= (1u << 24))
         return fail(FDK_ERR_UNSUPPORTED, "backproject: projection larger than 4 GiB");
     a.ang = (const R *)L.ang;
     a.scales = (const R *)L.scales;
@@ -335,7 +342,8 @@ static int launch_bp_t(const BPLaunch &L, hipStream_t st)
     if (nb == 0) return FDK_OK;
     if (nb > 0x7fffffffLL) return fail(FDK_ERR_UNSUPPORTED, "backproject: volume too large");
     a.nblocks = (int)nb;
-    hipLaunchKernelGGL((bp_kernel<Cell, ZR, (ZR < 8 ? ZR : 8)>), dim3((unsigned)nb), dim3(256), 0, st, a);
+    hipLaunchKernelGGL((bp_kernel<Q, ZR, (ZR < TKV_BP_LB ? ZR : TKV_BP_LB)>), dim3((unsigned)nb),
+                       dim3(256), 0, st, a);
     TKV_HIP(hipGetLastError());
     return FDK_OK;
 }
@@ -343,10 +351,10 @@ static int launch_bp_t(const BPLaunch &L, hipStream_t st)
 int launch_backproject(int cell_dtype, const BPLaunch &a, hipStream_t st)
 {
     switch (cell_dtype) {
-    case FDK_F32: return launch_bp_t<float2>(a, st);
-    case FDK_F16: return launch_bp_t<half2u>(a, st);
-    case FDK_BF16: return launch_bp_t<bf162u>(a, st);
-    case FDK_F64: return launch_bp_t<double2>(a, st);
+    case FDK_F32: return launch_bp_t<quad_f32>(a, st);
+    case FDK_F16: return launch_bp_t<quad_f16>(a, st);
+    case FDK_BF16: return launch_bp_t<quad_bf16>(a, st);
+    case FDK_F64: return launch_bp_t<quad_f64>(a, st);
     default: return fail(FDK_ERR_INVALID_ARGUMENT, "backproject: unknown cell type");
     }
 }

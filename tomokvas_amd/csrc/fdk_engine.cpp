@@ -107,9 +107,11 @@ int dalloc(void **p, size_t bytes, const char *what)
     return FDK_OK;
 }
 
-hipStream_t pick(fdk_ctx *c, void *stream)
+// stage-level calls run on the caller's stream; NULL is the default stream
+// (ordered with the caller's other work), never the engine's private stream
+hipStream_t pick(fdk_ctx *, void *stream)
 {
-    return stream ? (hipStream_t)stream : c->stream;
+    return (hipStream_t)stream;
 }
 
 int check_geometry(const tkv::Geometry &g)
@@ -146,8 +148,8 @@ int prepare(fdk_ctx *c)
     free_tables(c);
     const tkv::Geometry &g = c->g;
     c->p = tkv::padded_length(g.Nu);
-    // z tables cover every z-run of every kernel (largest run 16) plus slack
-    c->nz_pad = ((g.Nz + 15) / 16) * 16 + 16;
+    // z tables cover every z-run of every kernel (largest run 64) plus slack
+    c->nz_pad = ((g.Nz + 63) / 64) * 64 + 64;
 
     std::vector<double> s, co;
     tkv::angle_tables(g, s, co);
@@ -209,20 +211,11 @@ int need_prepared(fdk_ctx *c)
     return set_device(c);
 }
 
-int cell_bytes(int dtype)
-{
-    switch (dtype) {
-    case FDK_F32: return 8;
-    case FDK_F16: return 4;
-    case FDK_BF16: return 4;
-    case FDK_F64: return 16;
-    default: return 0;
-    }
-}
+int cell_bytes(int dtype) { return tkv::cell_bytes_of(dtype); }
 
-// cell rows are padded to a multiple of 64 cells with at least one zero
-// column (the right-hand tap of u = Nu-1 has weight 0 but is loaded)
-int cell_pitch(int Nu) { return ((Nu + 1 + 63) / 64) * 64; }
+// cell rows are padded to a multiple of 64 cells (256-byte aligned rows for
+// FP16 quads); the padding cells are zero
+int cell_pitch(int Nu) { return ((Nu + 63) / 64) * 64; }
 
 double out_scale(const fdk_ctx *c, int dtype)
 {

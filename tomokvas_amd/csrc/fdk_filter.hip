@@ -30,18 +30,22 @@ __device__ __forceinline__ cpx<R> cmul(cpx<R> a, cpx<R> b)
     return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
 }
 
-// Cosine weight of one detector sample in the reference's float operations
-// (kernel.cu:187, :197-198): D / sqrt(ce^2 + D^2 + cs^2).
+// Cosine weight of one detector sample (kernel.cu:187, :197-198):
+// D / sqrt(ce^2 + D^2 + cs^2), the sum contracted left to right into fmas as
+// nvcc compiles the reference expression.
+template <typename R>
+__device__ __forceinline__ R rfma_w(R a, R b, R c);
+template <>
+__device__ __forceinline__ float rfma_w(float a, float b, float c) { return fmaf(a, b, c); }
+template <>
+__device__ __forceinline__ double rfma_w(double a, double b, double c) { return fma(a, b, c); }
+
 template <typename R>
 __device__ __forceinline__ R cos_weight(int u, int v, R cX, R cZ, R D)
 {
-#pragma clang fp contract(off)
     R ce = (R)u - cX;
     R cs = (R)v - cZ;
-    R d2 = D * D;
-    R t = ce * ce;
-    t = t + d2;
-    t = t + cs * cs;
+    R t = rfma_w(cs, cs, rfma_w(ce, ce, D * D));
     return D / sqrt(t);
 }
 
@@ -434,48 +438,61 @@ int launch_tap_gradient(int real_bytes, const TapGradLaunch &a, hipStream_t st)
 }
 
 // ---------------------------------------------------------------- repack to cells
-// filtered [v][beta][u] (R)  ->  cells [beta][v][pitch]:
-//   cell(beta, v, u) = (Q[v][beta][u], Q[v+1][beta][u]) / scale_beta
-// Each thread walks VB consecutive rows of one column, reading each sample once.
+// filtered [v][beta][u] (R)  ->  quad cells [beta][v][pitch] of the bilinear
+// coefficients of the backprojector (Q = 0 outside [0,Nv) x [0,Nu)):
+//   A = Q(v,u), B = Q(v+1,u) - Q(v,u), C = Q(v,u+1) - Q(v,u),
+//   D = (Q(v+1,u+1) - Q(v+1,u)) - (Q(v,u+1) - Q(v,u)),   all divided by scale_beta.
+// Each thread walks PACK_VB consecutive rows of one column.
 template <typename Cell>
-struct CellTraits;
+struct QuadMake;
 template <>
-struct CellTraits<float2> {
-    static __device__ __forceinline__ float2 make(float a, float b) { return {a, b}; }
-};
-template <>
-struct CellTraits<double2> {
-    static __device__ __forceinline__ double2 make(double a, double b) { return {a, b}; }
-};
-struct half2u {
-    unsigned v;
-};
-struct bf162u {
-    unsigned v;
-};
-template <>
-struct CellTraits<half2u> {
-    static __device__ __forceinline__ half2u make(float a, float b)
+struct QuadMake<float4> {
+    static __device__ __forceinline__ float4 make(float a, float b, float c, float d)
     {
-        unsigned lo = __half_as_ushort(__float2half_rn(a));
-        unsigned hi = __half_as_ushort(__float2half_rn(b));
-        return {lo | (hi << 16)};
+        return {a, b, c, d};
     }
 };
 template <>
-struct CellTraits<bf162u> {
-    static __device__ __forceinline__ bf162u make(float a, float b)
+struct QuadMake<double4> {
+    static __device__ __forceinline__ double4 make(double a, double b, double c, double d)
     {
-        unsigned lo = __bfloat16_as_ushort(__float2bfloat16(a));
-        unsigned hi = __bfloat16_as_ushort(__float2bfloat16(b));
-        return {lo | (hi << 16)};
+        return {a, b, c, d};
+    }
+};
+struct quad_h {
+    uint2 v;
+};
+struct quad_b {
+    uint2 v;
+};
+template <>
+struct QuadMake<quad_h> {
+    static __device__ __forceinline__ unsigned h(float x)
+    {
+        return __half_as_ushort(__float2half_rn(x));
+    }
+    static __device__ __forceinline__ quad_h make(float a, float b, float c, float d)
+    {
+        return {{h(a) | (h(b) << 16), h(c) | (h(d) << 16)}};
+    }
+};
+template <>
+struct QuadMake<quad_b> {
+    static __device__ __forceinline__ unsigned h(float x)
+    {
+        return __bfloat16_as_ushort(__float2bfloat16(x));
+    }
+    static __device__ __forceinline__ quad_b make(float a, float b, float c, float d)
+    {
+        return {{h(a) | (h(b) << 16), h(c) | (h(d) << 16)}};
     }
 };
 
 constexpr int PACK_VB = 16;
-// FP16 keeps the largest filtered magnitude of a projection at 2^14, far from
-// the FP16 maximum 65504 and 2^28 above its smallest normal number.
-constexpr float F16_TARGET = 16384.0f;
+// FP16: the largest filtered magnitude of a projection maps to 2^13, so the
+// second difference D (at most 4x that) stays below the FP16 maximum 65504,
+// and values 2^27 below the maximum are still normal numbers.
+constexpr float F16_TARGET = 8192.0f;
 
 template <typename R, typename Cell, bool SCALED>
 __global__ void __launch_bounds__(256)
@@ -494,16 +511,22 @@ __global__ void __launch_bounds__(256)
         inv = (R)1 / sc;
     }
     if (u == 0 && blockIdx.y == 0) scales[beta] = sc;
-    const bool col = u < Nu;
     const size_t rstride = (size_t)nang * Nu;
-    const R *src = filt + (size_t)beta * Nu + u;
+    const R *src = filt + (size_t)beta * Nu;
+    const bool c0 = u < Nu, c1 = u + 1 < Nu;
     Cell *dst = cells + ((size_t)beta * Nv) * pitch + u;
-    R prev = (col && v0 < Nv) ? src[(size_t)v0 * rstride] : (R)0;
+    R q00 = (c0 && v0 < Nv) ? src[(size_t)v0 * rstride + u] : (R)0;
+    R q01 = (c1 && v0 < Nv) ? src[(size_t)v0 * rstride + u + 1] : (R)0;
     const int vend = min(v0 + PACK_VB, Nv);
     for (int v = v0; v < vend; ++v) {
-        R next = (col && v + 1 < Nv) ? src[(size_t)(v + 1) * rstride] : (R)0;
-        dst[(size_t)v * pitch] = CellTraits<Cell>::make(prev * inv, next * inv);
-        prev = next;
+        const bool rn = v + 1 < Nv;
+        R q10 = (c0 && rn) ? src[(size_t)(v + 1) * rstride + u] : (R)0;
+        R q11 = (c1 && rn) ? src[(size_t)(v + 1) * rstride + u + 1] : (R)0;
+        R dB = q10 - q00, dC = q01 - q00;
+        R dD = (q11 - q10) - dC;
+        dst[(size_t)v * pitch] = QuadMake<Cell>::make(q00 * inv, dB * inv, dC * inv, dD * inv);
+        q00 = q10;
+        q01 = q11;
     }
 }
 
@@ -517,24 +540,24 @@ int launch_pack(int dtype_in, const void *filt, int Nu, int Nv, int nang, int pi
     if (dtype_in == FDK_F64) {
         if (cell_dtype != FDK_F64)
             return fail(FDK_ERR_INVALID_ARGUMENT, "pack: F64 input needs F64 cells");
-        hipLaunchKernelGGL((pack_kernel<double, double2, false>), grid, dim3(256), 0, st,
-                           (const double *)filt, Nu, Nv, nang, pitch, maxabs, (double2 *)cells,
+        hipLaunchKernelGGL((pack_kernel<double, double4, false>), grid, dim3(256), 0, st,
+                           (const double *)filt, Nu, Nv, nang, pitch, maxabs, (double4 *)cells,
                            (double *)scales);
     } else {
         const float *f = (const float *)filt;
         switch (cell_dtype) {
         case FDK_F32:
-            hipLaunchKernelGGL((pack_kernel<float, float2, false>), grid, dim3(256), 0, st, f,
-                               Nu, Nv, nang, pitch, maxabs, (float2 *)cells, (float *)scales);
+            hipLaunchKernelGGL((pack_kernel<float, float4, false>), grid, dim3(256), 0, st, f,
+                               Nu, Nv, nang, pitch, maxabs, (float4 *)cells, (float *)scales);
             break;
         case FDK_F16:
             if (!maxabs) return fail(FDK_ERR_INVALID_ARGUMENT, "pack: FP16 needs maxabs");
-            hipLaunchKernelGGL((pack_kernel<float, half2u, true>), grid, dim3(256), 0, st, f,
-                               Nu, Nv, nang, pitch, maxabs, (half2u *)cells, (float *)scales);
+            hipLaunchKernelGGL((pack_kernel<float, quad_h, true>), grid, dim3(256), 0, st, f,
+                               Nu, Nv, nang, pitch, maxabs, (quad_h *)cells, (float *)scales);
             break;
         case FDK_BF16:
-            hipLaunchKernelGGL((pack_kernel<float, bf162u, false>), grid, dim3(256), 0, st, f,
-                               Nu, Nv, nang, pitch, maxabs, (bf162u *)cells, (float *)scales);
+            hipLaunchKernelGGL((pack_kernel<float, quad_b, false>), grid, dim3(256), 0, st, f,
+                               Nu, Nv, nang, pitch, maxabs, (quad_b *)cells, (float *)scales);
             break;
         default:
             return fail(FDK_ERR_INVALID_ARGUMENT, "pack: F32 input needs F32/F16/BF16 cells");

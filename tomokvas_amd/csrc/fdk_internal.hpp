@@ -4,10 +4,10 @@
 // configuration without tiling):
 //   input stack   float [v][beta][u]          as the host hands it over
 //                 (sinogram-major, FBPG/mucat_file.cpp:244-267)
-//   cell stack    Cell  [beta][v][pitch]      filtered projections, angle-major;
-//                 cell(beta,v,u) = (Q[beta][v][u], Q[beta][v+1][u]) so one
-//                 load of two adjacent cells returns the whole 2x2 bilinear
-//                 neighbourhood; row Nv and columns >= Nu are zero.
+//   cell stack    Quad  [beta][v][pitch]      filtered projections, angle-major;
+//                 quad(beta,v,u) = the four bilinear coefficients of the
+//                 2x2 neighbourhood at (v,u) (fdk_filter.hip, pack_kernel),
+//                 so one aligned load serves one voxel-update.
 //   volume        float [z][y][x]             the full volume, slabs are z-ranges
 #pragma once
 
@@ -98,6 +98,7 @@ struct BPLaunch {
 };
 int launch_backproject(int cell_dtype, const BPLaunch &a, hipStream_t st);
 int bp_zrun(int cell_dtype);
+int cell_bytes_of(int cell_dtype);   // bytes of one quad cell
 
 struct FPLaunch {           // adjoint of the backprojector
     const void *vol;        // [Nz][Ny][Nx]

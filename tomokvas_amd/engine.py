@@ -161,13 +161,13 @@ class FilteredBackProjection:
 
         def get_block(user, n, collecting):
             if sink is None:
-                return C.cast(None, C.POINTER(C.c_float))
+                return None
             buf = sink.GetBlock(n, bool(collecting))
             arr = np.ascontiguousarray(buf, dtype=np.float32)
-            if arr is not buf:
-                return C.cast(None, C.POINTER(C.c_float))
+            if arr is not buf:        # the engine must write into the sink's own memory
+                return None
             keep["buf"] = arr
-            return arr.ctypes.data_as(C.POINTER(C.c_float))
+            return arr.ctypes.data
 
         cb = _lib.GET_BLOCK_FN(get_block)
         self._cb = cb
