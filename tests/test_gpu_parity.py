@@ -311,3 +311,25 @@ def test_cfg2_full_size_properties(torch):
     eb = O.rel_rmse(got, ref_pts)
     say(f"  BP voxel sample vs FP64 oracle rel RMSE {eb:.3e}")
     assert eb <= 1e-5
+
+
+# ------------------------------------------------------------------ committed fixture
+def test_engine_matches_golden_fixture(torch):
+    import os
+    fx = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "fixtures.npz"))
+    keys = ["Nu", "Nang", "Nv", "cX", "cZ", "dbeta", "D", "tau", "Nx", "Ny", "Nz", "cx", "cy",
+            "cz", "rot", "scale", "flipX", "flipY", "flipZ", "B"]
+    d = dict(zip(keys, fx["fdk_geom"]))
+    g = scan(int(d["Nu"]), int(d["Nv"]), int(d["Nang"]), int(d["Nx"]), int(d["Ny"]), int(d["Nz"]),
+             D=d["D"], tau=d["tau"], scale=d["scale"], B=int(d["B"]))
+    vol, _, _ = run_engine(g, fx["fdk_sino"])
+    e = O.rel_rmse(vol, fx["fdk_vol64"])
+    say(f"  golden 32^3 FDK: FP32 engine vs FP64 fixture {e:.3e}")
+    assert e <= TOL_FDK
+    # FP64 stage path reproduces the fixture to FP64 rounding
+    st = stages(g)
+    s = torch.from_numpy(fx["fdk_sino"].astype(np.float64)).cuda()
+    st.filter(s, weight=True)
+    cells, scales = st.pack(s, _lib.F64)
+    v64 = st.backproject(cells, _lib.F64, scales).cpu().numpy()
+    assert O.rel_rmse(v64, fx["fdk_vol64"]) <= 1e-12
