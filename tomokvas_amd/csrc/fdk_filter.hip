@@ -187,21 +187,222 @@ __device__ __forceinline__ R group_max(R m, int width)
 __device__ __forceinline__ unsigned float_bits_abs(float m) { return __float_as_uint(m); }
 __device__ __forceinline__ unsigned float_bits_abs(double m) { return __float_as_uint((float)m); }
 
+// ---------------------------------------------------------------- radix-16 row filter
+// Mixed-radix Stockham plan: radix-16 passes, with the one radix-2/4/8 pass
+// that log2(p) mod 4 requires placed second, so the first and the last pass
+// are both radix 16 whenever p >= 256.  Every thread holds E = P/TPT complex
+// values (16; 32 for p = 8192) through all passes.
+template <int P>
+struct Plan {
+    static constexpr int LOG2 = __builtin_ctz(P);
+    static constexpr int N16 = LOG2 / 4;
+    static constexpr int REM = 1 << (LOG2 % 4);
+    static constexpr int NPASS = N16 == 0 ? 1 : N16 + (REM > 1 ? 1 : 0);
+    static constexpr int TPT = P < 16 ? 1 : (P / 16 < 256 ? P / 16 : 256);
+    static constexpr int NT = 256 / TPT;
+    static constexpr int E = P / TPT;
+    static constexpr int radix(int pass)
+    {
+        return N16 == 0 ? REM : (pass == 0 ? 16 : (REM > 1 && pass == 1 ? REM : 16));
+    }
+    static constexpr int RFIRST = radix(0);
+    static constexpr int RLAST = radix(NPASS - 1);
+};
+
+// LDS index with one pad slot per 16 values (bank-conflict relief), when the
+// padded transforms still fit 64 KiB per workgroup
+template <typename R, int P>
+struct Pad {
+    static constexpr bool ON = sizeof(cpx<R>) * (size_t)Plan<P>::NT * (P + P / 16) <= 65536;
+    static constexpr int SIZE = ON ? P + P / 16 : P;
+    static __device__ __forceinline__ int at(int n) { return ON ? n + (n >> 4) : n; }
+};
+
+// forward twiddle W_N^q = exp(-2 pi i q / N), N in {8, 16}: angles are
+// multiples of pi/8; cos and sin of k*pi/8 for k = 0..3 (the rest by symmetry)
+template <typename R, int N, int Q, bool INV>
+__device__ __forceinline__ cpx<R> wconst()
+{
+    constexpr int k = (Q * (16 / N)) % 16;        // angle k*pi/8, k in [0,16)
+    constexpr double C[4] = {1.0, 0.92387953251128675613, 0.70710678118654752440,
+                             0.38268343236508977173};
+    constexpr int q = k % 4, quad = k / 4;        // k*pi/8 = quad*pi/2 + q*pi/8
+    constexpr double c0 = C[q], s0 = q == 0 ? 0.0 : C[4 - q];
+    constexpr double c = quad == 0 ? c0 : quad == 1 ? -s0 : quad == 2 ? -c0 : s0;
+    constexpr double s = quad == 0 ? s0 : quad == 1 ? c0 : quad == 2 ? -s0 : -c0;
+    return {(R)c, (R)(INV ? s : -s)};
+}
+
+template <typename R, bool INV>
+__device__ __forceinline__ void dft4v(cpx<R> &a0, cpx<R> &a1, cpx<R> &a2, cpx<R> &a3)
+{
+    cpx<R> v0 = {a0.x + a2.x, a0.y + a2.y}, v1 = {a0.x - a2.x, a0.y - a2.y};
+    cpx<R> v2 = {a1.x + a3.x, a1.y + a3.y}, d = {a1.x - a3.x, a1.y - a3.y};
+    cpx<R> v3 = INV ? cpx<R>{-d.y, d.x} : cpx<R>{d.y, -d.x};
+    a0 = {v0.x + v2.x, v0.y + v2.y};
+    a1 = {v1.x + v3.x, v1.y + v3.y};
+    a2 = {v0.x - v2.x, v0.y - v2.y};
+    a3 = {v1.x - v3.x, v1.y - v3.y};
+}
+
+// In-register DFT of length RAD on x[0..RAD): X[k] = sum_n x[n] W^(nk).
+// RAD = 8, 16 split as N = 4*N2 with n = N2*n1 + n2, k = k1 + 4*k2 (inner
+// length-4 DFTs over n1, twiddles W_N^(n2 k1), outer length-N2 DFTs over n2).
+template <typename R, int RAD, bool INV>
+struct DFT;
+template <typename R, bool INV>
+struct DFT<R, 2, INV> {
+    static __device__ __forceinline__ void run(cpx<R> *x)
+    {
+        cpx<R> t = x[0];
+        x[0] = {t.x + x[1].x, t.y + x[1].y};
+        x[1] = {t.x - x[1].x, t.y - x[1].y};
+    }
+};
+template <typename R, bool INV>
+struct DFT<R, 4, INV> {
+    static __device__ __forceinline__ void run(cpx<R> *x) { dft4v<R, INV>(x[0], x[1], x[2], x[3]); }
+};
+template <typename R, bool INV>
+struct DFT<R, 8, INV> {
+    static __device__ __forceinline__ void run(cpx<R> *x)
+    {
+        cpx<R> y[2][4];
+#pragma unroll
+        for (int n2 = 0; n2 < 2; ++n2) {
+#pragma unroll
+            for (int n1 = 0; n1 < 4; ++n1) y[n2][n1] = x[2 * n1 + n2];
+            dft4v<R, INV>(y[n2][0], y[n2][1], y[n2][2], y[n2][3]);
+        }
+        y[1][1] = cmul(y[1][1], wconst<R, 8, 1, INV>());
+        y[1][2] = cmul(y[1][2], wconst<R, 8, 2, INV>());
+        y[1][3] = cmul(y[1][3], wconst<R, 8, 3, INV>());
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) {
+            x[k1] = {y[0][k1].x + y[1][k1].x, y[0][k1].y + y[1][k1].y};
+            x[k1 + 4] = {y[0][k1].x - y[1][k1].x, y[0][k1].y - y[1][k1].y};
+        }
+    }
+};
+template <typename R, bool INV>
+struct DFT<R, 16, INV> {
+    static __device__ __forceinline__ void run(cpx<R> *x)
+    {
+        cpx<R> y[4][4];
+#pragma unroll
+        for (int n2 = 0; n2 < 4; ++n2) {
+#pragma unroll
+            for (int n1 = 0; n1 < 4; ++n1) y[n2][n1] = x[4 * n1 + n2];
+            dft4v<R, INV>(y[n2][0], y[n2][1], y[n2][2], y[n2][3]);
+        }
+        y[1][1] = cmul(y[1][1], wconst<R, 16, 1, INV>());
+        y[1][2] = cmul(y[1][2], wconst<R, 16, 2, INV>());
+        y[1][3] = cmul(y[1][3], wconst<R, 16, 3, INV>());
+        y[2][1] = cmul(y[2][1], wconst<R, 16, 2, INV>());
+        y[2][2] = cmul(y[2][2], wconst<R, 16, 4, INV>());
+        y[2][3] = cmul(y[2][3], wconst<R, 16, 6, INV>());
+        y[3][1] = cmul(y[3][1], wconst<R, 16, 3, INV>());
+        y[3][2] = cmul(y[3][2], wconst<R, 16, 6, INV>());
+        y[3][3] = cmul(y[3][3], wconst<R, 16, 9, INV>());
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) {
+            dft4v<R, INV>(y[0][k1], y[1][k1], y[2][k1], y[3][k1]);
+#pragma unroll
+            for (int k2 = 0; k2 < 4; ++k2) x[k1 + 4 * k2] = y[k2][k1];
+        }
+    }
+};
+
+// one Stockham pass on the thread's registers: butterfly i = t + b*TPT holds
+// its inputs (indices i + m*P/RAD) in reg[b*RAD + m]; outputs belong at
+// ((i - k)*RAD + k) + m*pc with k = i mod pc
+template <typename R, int P, int RAD, bool INV>
+__device__ __forceinline__ void pass_compute(cpx<R> *reg, const cpx<R> *__restrict__ tw, int t,
+                                             int pc)
+{
+    using PL = Plan<P>;
+#pragma unroll
+    for (int b = 0; b < PL::E / RAD; ++b) {
+        const int i = t + b * PL::TPT;
+        const int k = i & (pc - 1);
+        cpx<R> *x = reg + b * RAD;
+        if (pc > 1) {
+            const int stride = P / (RAD * pc);
+#pragma unroll
+            for (int m = 1; m < RAD; ++m) {
+                cpx<R> w = tw[m * k * stride];
+                if (INV) w.y = -w.y;
+                x[m] = cmul(x[m], w);
+            }
+        }
+        DFT<R, RAD, INV>::run(x);
+    }
+}
+
+template <typename R, int P, int RAD>
+__device__ __forceinline__ void pass_gather(cpx<R> *reg, const cpx<R> *s, int t)
+{
+    using PL = Plan<P>;
+#pragma unroll
+    for (int b = 0; b < PL::E / RAD; ++b) {
+        const int i = t + b * PL::TPT;
+#pragma unroll
+        for (int m = 0; m < RAD; ++m) reg[b * RAD + m] = s[Pad<R, P>::at(i + m * (P / RAD))];
+    }
+}
+
+template <typename R, int P, int RAD>
+__device__ __forceinline__ void pass_scatter(const cpx<R> *reg, cpx<R> *s, int t, int pc)
+{
+    using PL = Plan<P>;
+#pragma unroll
+    for (int b = 0; b < PL::E / RAD; ++b) {
+        const int i = t + b * PL::TPT;
+        const int k = i & (pc - 1);
+        const int j = (i - k) * RAD + k;
+#pragma unroll
+        for (int m = 0; m < RAD; ++m) s[Pad<R, P>::at(j + m * pc)] = reg[b * RAD + m];
+    }
+}
+
+// all passes of one transform; on entry reg holds the first pass's inputs, on
+// exit the outputs X[i + m*P/RLAST] of butterfly i = t + b*TPT in reg[b*RLAST + m]
+template <typename R, int P, bool INV, int PASS>
+struct Passes {
+    static __device__ __forceinline__ void run(cpx<R> *reg, cpx<R> *s,
+                                               const cpx<R> *__restrict__ tw, int t, int pc)
+    {
+        using PL = Plan<P>;
+        constexpr int RAD = PL::radix(PASS);
+        if (PASS > 0) {
+            __syncthreads();   // the previous pass's outputs are in LDS
+            pass_gather<R, P, RAD>(reg, s, t);
+        }
+        pass_compute<R, P, RAD, INV>(reg, tw, t, pc);
+        if (PASS + 1 < PL::NPASS) {
+            __syncthreads();   // every thread has read its inputs
+            pass_scatter<R, P, RAD>(reg, s, t, pc);
+            Passes<R, P, INV, (PASS + 1 < PL::NPASS ? PASS + 1 : PASS)>::run(reg, s, tw, t,
+                                                                           pc * RAD);
+        }
+    }
+};
+
 template <typename R, int P>
 __global__ void __launch_bounds__(256)
-    filter_kernel(const R *in, R *out, int Nu, int Nv, int nang, int weight,
-                  R cX, R cZ, R D, const R *__restrict__ Hre, const R *__restrict__ Him,
-                  int conj, const cpx<R> *__restrict__ tw, unsigned *__restrict__ maxabs)
+    filter_kernel(const R *in, R *out, int Nu, int Nv, int nang, int weight, R cX, R cZ, R D,
+                  const R *__restrict__ Hre, const R *__restrict__ Him, int conj,
+                  const cpx<R> *__restrict__ tw, unsigned *__restrict__ maxabs)
 {
-    using S = FFTShape<P>;
+    using PL = Plan<P>;
+    constexpr int E = PL::E;
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    cpx<R> *smem = reinterpret_cast<cpx<R> *>(smem_raw);
     const int tid = threadIdx.x;
-    const int f = tid / S::TPT;           // transform within the workgroup
-    const int t = tid - f * S::TPT;       // thread within the transform
-    cpx<R> *s = smem + f * P;
+    const int f = tid / PL::TPT;          // transform within the workgroup
+    const int t = tid - f * PL::TPT;      // thread within the transform
+    cpx<R> *s = reinterpret_cast<cpx<R> *>(smem_raw) + f * Pad<R, P>::SIZE;
     const long long rows = (long long)Nv * nang;
-    const long long rA = 2 * ((long long)blockIdx.x * S::NT + f);
+    const long long rA = 2 * ((long long)blockIdx.x * PL::NT + f);
     const long long rB = rA + 1;
     const bool hasA = rA < rows, hasB = rB < rows;
     const int vA = hasA ? (int)(rA / nang) : 0;
@@ -209,62 +410,86 @@ __global__ void __launch_bounds__(256)
     const R *inA = in + (hasA ? rA : 0) * Nu;
     const R *inB = in + (hasB ? rB : 0) * Nu;
 
-    // load two rows (coalesced along u), optional fused cosine weight, zero pad
+    cpx<R> reg[E];
+    // first pass inputs straight from HBM: two rows as (re, im), optional
+    // fused cosine weight, zero padding beyond Nu
 #pragma unroll
-    for (int j = 0; j < P / S::TPT; ++j) {
-        int n = t + j * S::TPT;
-        R a = 0, b = 0;
-        if (n < Nu) {
-            if (hasA) a = inA[n];
-            if (hasB) b = inB[n];
-            if (weight) {
-                a *= cos_weight<R>(n, vA, cX, cZ, D);
-                b *= cos_weight<R>(n, vB, cX, cZ, D);
+    for (int b = 0; b < E / PL::RFIRST; ++b) {
+        const int i = t + b * PL::TPT;
+#pragma unroll
+        for (int m = 0; m < PL::RFIRST; ++m) {
+            const int n = i + m * (P / PL::RFIRST);
+            R a = 0, c = 0;
+            if (n < Nu) {
+                if (hasA) a = inA[n];
+                if (hasB) c = inB[n];
+                if (weight) {
+                    a *= cos_weight<R>(n, vA, cX, cZ, D);
+                    c *= cos_weight<R>(n, vB, cX, cZ, D);
+                }
+            }
+            reg[b * PL::RFIRST + m] = {a, c};
+        }
+    }
+    Passes<R, P, false, 0>::run(reg, s, tw, t, 1);
+
+    // bin-wise product with the spectrum (FiltrationKernel, kernel.cu:40-53), in
+    // registers: reg[b*RLAST + m] holds X[i + m*P/RLAST]
+#pragma unroll
+    for (int b = 0; b < E / PL::RLAST; ++b) {
+        const int i = t + b * PL::TPT;
+#pragma unroll
+        for (int m = 0; m < PL::RLAST; ++m) {
+            const int n = i + m * (P / PL::RLAST);
+            cpx<R> z = reg[b * PL::RLAST + m];
+            R hr = Hre[n];
+            if (Him) {
+                R hi = conj ? -Him[n] : Him[n];
+                reg[b * PL::RLAST + m] = {z.x * hr - z.y * hi, z.x * hi + z.y * hr};
+            } else {
+                reg[b * PL::RLAST + m] = {z.x * hr, z.y * hr};
             }
         }
-        s[n] = {a, b};
     }
-    __syncthreads();
-
-    fft_lds<R, P, false>(s, tw, t);
-
-    // bin-wise product with the spectrum (FiltrationKernel, kernel.cu:40-53)
+    if (PL::RFIRST != PL::RLAST) {
+        // hand the spectrum to the inverse transform's first pass through LDS
+        __syncthreads();
 #pragma unroll
-    for (int j = 0; j < P / S::TPT; ++j) {
-        int n = t + j * S::TPT;
-        cpx<R> z = s[n];
-        R hr = Hre[n];
-        if (Him) {
-            R hi = conj ? -Him[n] : Him[n];
-            s[n] = {z.x * hr - z.y * hi, z.x * hi + z.y * hr};
-        } else {
-            s[n] = {z.x * hr, z.y * hr};
+        for (int b = 0; b < E / PL::RLAST; ++b) {
+            const int i = t + b * PL::TPT;
+#pragma unroll
+            for (int m = 0; m < PL::RLAST; ++m)
+                s[Pad<R, P>::at(i + m * (P / PL::RLAST))] = reg[b * PL::RLAST + m];
         }
+        __syncthreads();
+        pass_gather<R, P, PL::RFIRST>(reg, s, t);
     }
-    __syncthreads();
+    Passes<R, P, true, 0>::run(reg, s, tw, t, 1);
 
-    fft_lds<R, P, true>(s, tw, t);
-
-    // crop to Nu, write back, per-projection max |value|
+    // crop to Nu straight to HBM, per-projection max |value|
     R mA = 0, mB = 0;
     R *outA = out + (hasA ? rA : 0) * Nu;
     R *outB = out + (hasB ? rB : 0) * Nu;
 #pragma unroll
-    for (int j = 0; j < P / S::TPT; ++j) {
-        int n = t + j * S::TPT;
-        if (n < Nu) {
-            cpx<R> z = s[n];
-            if (hasA) outA[n] = z.x;
-            if (hasB) outB[n] = z.y;
-            R ax = z.x < 0 ? -z.x : z.x, ay = z.y < 0 ? -z.y : z.y;
-            mA = ax > mA ? ax : mA;
-            mB = ay > mB ? ay : mB;
+    for (int b = 0; b < E / PL::RLAST; ++b) {
+        const int i = t + b * PL::TPT;
+#pragma unroll
+        for (int m = 0; m < PL::RLAST; ++m) {
+            const int n = i + m * (P / PL::RLAST);
+            if (n < Nu) {
+                cpx<R> z = reg[b * PL::RLAST + m];
+                if (hasA) outA[n] = z.x;
+                if (hasB) outB[n] = z.y;
+                R ax = z.x < 0 ? -z.x : z.x, ay = z.y < 0 ? -z.y : z.y;
+                mA = ax > mA ? ax : mA;
+                mB = ay > mB ? ay : mB;
+            }
         }
     }
     if (maxabs) {
-        mA = group_max(mA, S::TPT);
-        mB = group_max(mB, S::TPT);
-        bool leader = (S::TPT >= 64) ? ((tid & 63) == 0) : (t == 0);
+        mA = group_max(mA, PL::TPT);
+        mB = group_max(mB, PL::TPT);
+        bool leader = (PL::TPT >= 64) ? ((tid & 63) == 0) : (t == 0);
         if (leader) {
             if (hasA) atomicMax(&maxabs[rA % nang], float_bits_abs(mA));
             if (hasB) atomicMax(&maxabs[rB % nang], float_bits_abs(mB));
@@ -275,17 +500,17 @@ __global__ void __launch_bounds__(256)
 template <typename R, int P>
 static int launch_filter_p(const FilterLaunch &a, hipStream_t st)
 {
-    using S = FFTShape<P>;
+    using PL = Plan<P>;
     long long rows = (long long)a.Nv * a.nang;
     long long pairs = (rows + 1) / 2;
-    long long blocks = (pairs + S::NT - 1) / S::NT;
+    long long blocks = (pairs + PL::NT - 1) / PL::NT;
     if (blocks == 0) return FDK_OK;
     if (blocks > 0x7fffffffLL) return fail(FDK_ERR_UNSUPPORTED, "filter: too many rows");
-    size_t lds = sizeof(cpx<R>) * (size_t)S::NT * P;
+    size_t lds = PL::NPASS > 1 || PL::RFIRST != PL::RLAST
+                     ? sizeof(cpx<R>) * (size_t)PL::NT * Pad<R, P>::SIZE
+                     : 0;
     auto kern = filter_kernel<R, P>;
-    if (lds > 65536) TKV_HIP(hipFuncSetAttribute((const void *)kern,
-                                                  hipFuncAttributeMaxDynamicSharedMemorySize,
-                                                  (int)lds));
+    if (lds > 65536) return fail(FDK_ERR_UNSUPPORTED, "filter: transform does not fit LDS");
     hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(256), lds, st, (const R *)a.in,
                        (R *)a.out, a.Nu, a.Nv, a.nang, a.weight, (R)a.cX, (R)a.cZ, (R)a.D,
                        (const R *)a.Hre, (const R *)a.Him, a.conj, (const cpx<R> *)a.tw,
