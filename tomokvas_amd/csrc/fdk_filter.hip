@@ -17,6 +17,11 @@
 #include <hip/hip_bf16.h>
 #include <hip/hip_fp16.h>
 
+// waves per SIMD the row filter is compiled for (2: no spills at p = 4096)
+#ifndef TKV_FILTER_WAVES
+#define TKV_FILTER_WAVES 2
+#endif
+
 namespace tkv {
 
 template <typename R>
@@ -245,13 +250,17 @@ __device__ __forceinline__ void dft4v(cpx<R> &a0, cpx<R> &a1, cpx<R> &a2, cpx<R>
     a3 = {v1.x - v3.x, v1.y - v3.y};
 }
 
-// In-register DFT of length RAD on x[0..RAD): X[k] = sum_n x[n] W^(nk).
-// RAD = 8, 16 split as N = 4*N2 with n = N2*n1 + n2, k = k1 + 4*k2 (inner
-// length-4 DFTs over n1, twiddles W_N^(n2 k1), outer length-N2 DFTs over n2).
+// In-register DFT of length RAD on x[0..RAD), in place: X[k] = sum_n x[n] W^(nk)
+// ends up in x[DFT::pos(k)].  RAD = 8, 16 are split as N = 4*N2 with
+// n = N2*n1 + n2, k = k1 + 4*k2: length-4 DFTs over n1 on the elements
+// {n2 + N2*n1} (in place, y(n2,k1) lands at n2 + N2*k1), twiddles W_N^(n2 k1),
+// then length-N2 DFTs over n2 on the contiguous groups {N2*k1 + n2}, leaving
+// X[k1 + 4*k2] at N2*k1 + k2.  No temporaries beyond the RAD values.
 template <typename R, int RAD, bool INV>
 struct DFT;
 template <typename R, bool INV>
 struct DFT<R, 2, INV> {
+    static constexpr int pos(int k) { return k; }
     static __device__ __forceinline__ void run(cpx<R> *x)
     {
         cpx<R> t = x[0];
@@ -261,55 +270,47 @@ struct DFT<R, 2, INV> {
 };
 template <typename R, bool INV>
 struct DFT<R, 4, INV> {
+    static constexpr int pos(int k) { return k; }
     static __device__ __forceinline__ void run(cpx<R> *x) { dft4v<R, INV>(x[0], x[1], x[2], x[3]); }
 };
 template <typename R, bool INV>
 struct DFT<R, 8, INV> {
+    static constexpr int pos(int k) { return 2 * (k % 4) + k / 4; }
     static __device__ __forceinline__ void run(cpx<R> *x)
     {
-        cpx<R> y[2][4];
-#pragma unroll
-        for (int n2 = 0; n2 < 2; ++n2) {
-#pragma unroll
-            for (int n1 = 0; n1 < 4; ++n1) y[n2][n1] = x[2 * n1 + n2];
-            dft4v<R, INV>(y[n2][0], y[n2][1], y[n2][2], y[n2][3]);
-        }
-        y[1][1] = cmul(y[1][1], wconst<R, 8, 1, INV>());
-        y[1][2] = cmul(y[1][2], wconst<R, 8, 2, INV>());
-        y[1][3] = cmul(y[1][3], wconst<R, 8, 3, INV>());
+        dft4v<R, INV>(x[0], x[2], x[4], x[6]);
+        dft4v<R, INV>(x[1], x[3], x[5], x[7]);
+        x[3] = cmul(x[3], wconst<R, 8, 1, INV>());
+        x[5] = cmul(x[5], wconst<R, 8, 2, INV>());
+        x[7] = cmul(x[7], wconst<R, 8, 3, INV>());
 #pragma unroll
         for (int k1 = 0; k1 < 4; ++k1) {
-            x[k1] = {y[0][k1].x + y[1][k1].x, y[0][k1].y + y[1][k1].y};
-            x[k1 + 4] = {y[0][k1].x - y[1][k1].x, y[0][k1].y - y[1][k1].y};
+            cpx<R> a = x[2 * k1], b = x[2 * k1 + 1];
+            x[2 * k1] = {a.x + b.x, a.y + b.y};
+            x[2 * k1 + 1] = {a.x - b.x, a.y - b.y};
         }
     }
 };
 template <typename R, bool INV>
 struct DFT<R, 16, INV> {
+    static constexpr int pos(int k) { return 4 * (k % 4) + k / 4; }
     static __device__ __forceinline__ void run(cpx<R> *x)
     {
-        cpx<R> y[4][4];
 #pragma unroll
-        for (int n2 = 0; n2 < 4; ++n2) {
+        for (int n2 = 0; n2 < 4; ++n2) dft4v<R, INV>(x[n2], x[n2 + 4], x[n2 + 8], x[n2 + 12]);
+        // y(n2, k1) at n2 + 4*k1 gets W16^(n2*k1)
+        x[5] = cmul(x[5], wconst<R, 16, 1, INV>());
+        x[9] = cmul(x[9], wconst<R, 16, 2, INV>());
+        x[13] = cmul(x[13], wconst<R, 16, 3, INV>());
+        x[6] = cmul(x[6], wconst<R, 16, 2, INV>());
+        x[10] = cmul(x[10], wconst<R, 16, 4, INV>());
+        x[14] = cmul(x[14], wconst<R, 16, 6, INV>());
+        x[7] = cmul(x[7], wconst<R, 16, 3, INV>());
+        x[11] = cmul(x[11], wconst<R, 16, 6, INV>());
+        x[15] = cmul(x[15], wconst<R, 16, 9, INV>());
 #pragma unroll
-            for (int n1 = 0; n1 < 4; ++n1) y[n2][n1] = x[4 * n1 + n2];
-            dft4v<R, INV>(y[n2][0], y[n2][1], y[n2][2], y[n2][3]);
-        }
-        y[1][1] = cmul(y[1][1], wconst<R, 16, 1, INV>());
-        y[1][2] = cmul(y[1][2], wconst<R, 16, 2, INV>());
-        y[1][3] = cmul(y[1][3], wconst<R, 16, 3, INV>());
-        y[2][1] = cmul(y[2][1], wconst<R, 16, 2, INV>());
-        y[2][2] = cmul(y[2][2], wconst<R, 16, 4, INV>());
-        y[2][3] = cmul(y[2][3], wconst<R, 16, 6, INV>());
-        y[3][1] = cmul(y[3][1], wconst<R, 16, 3, INV>());
-        y[3][2] = cmul(y[3][2], wconst<R, 16, 6, INV>());
-        y[3][3] = cmul(y[3][3], wconst<R, 16, 9, INV>());
-#pragma unroll
-        for (int k1 = 0; k1 < 4; ++k1) {
-            dft4v<R, INV>(y[0][k1], y[1][k1], y[2][k1], y[3][k1]);
-#pragma unroll
-            for (int k2 = 0; k2 < 4; ++k2) x[k1 + 4 * k2] = y[k2][k1];
-        }
+        for (int k1 = 0; k1 < 4; ++k1)
+            dft4v<R, INV>(x[4 * k1], x[4 * k1 + 1], x[4 * k1 + 2], x[4 * k1 + 3]);
     }
 };
 
@@ -361,12 +362,14 @@ __device__ __forceinline__ void pass_scatter(const cpx<R> *reg, cpx<R> *s, int t
         const int k = i & (pc - 1);
         const int j = (i - k) * RAD + k;
 #pragma unroll
-        for (int m = 0; m < RAD; ++m) s[Pad<R, P>::at(j + m * pc)] = reg[b * RAD + m];
+        for (int m = 0; m < RAD; ++m)
+            s[Pad<R, P>::at(j + m * pc)] = reg[b * RAD + DFT<R, RAD, false>::pos(m)];
     }
 }
 
 // all passes of one transform; on entry reg holds the first pass's inputs, on
-// exit the outputs X[i + m*P/RLAST] of butterfly i = t + b*TPT in reg[b*RLAST + m]
+// exit the outputs X[i + m*P/RLAST] of butterfly i = t + b*TPT in
+// reg[b*RLAST + DFT::pos(m)]
 template <typename R, int P, bool INV, int PASS>
 struct Passes {
     static __device__ __forceinline__ void run(cpx<R> *reg, cpx<R> *s,
@@ -389,7 +392,7 @@ struct Passes {
 };
 
 template <typename R, int P>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TKV_FILTER_WAVES)))
     filter_kernel(const R *in, R *out, int Nu, int Nv, int nang, int weight, R cX, R cZ, R D,
                   const R *__restrict__ Hre, const R *__restrict__ Him, int conj,
                   const cpx<R> *__restrict__ tw, unsigned *__restrict__ maxabs)
@@ -412,7 +415,9 @@ __global__ void __launch_bounds__(256)
 
     cpx<R> reg[E];
     // first pass inputs straight from HBM: two rows as (re, im), optional
-    // fused cosine weight, zero padding beyond Nu
+    // fused cosine weight (one evaluation when both rows share a detector
+    // row), zero padding beyond Nu
+    const bool same_v = vA == vB;
 #pragma unroll
     for (int b = 0; b < E / PL::RFIRST; ++b) {
         const int i = t + b * PL::TPT;
@@ -424,8 +429,10 @@ __global__ void __launch_bounds__(256)
                 if (hasA) a = inA[n];
                 if (hasB) c = inB[n];
                 if (weight) {
-                    a *= cos_weight<R>(n, vA, cX, cZ, D);
-                    c *= cos_weight<R>(n, vB, cX, cZ, D);
+                    R wa = cos_weight<R>(n, vA, cX, cZ, D);
+                    R wb = same_v ? wa : cos_weight<R>(n, vB, cX, cZ, D);
+                    a *= wa;
+                    c *= wb;
                 }
             }
             reg[b * PL::RFIRST + m] = {a, c};
@@ -434,23 +441,28 @@ __global__ void __launch_bounds__(256)
     Passes<R, P, false, 0>::run(reg, s, tw, t, 1);
 
     // bin-wise product with the spectrum (FiltrationKernel, kernel.cu:40-53), in
-    // registers: reg[b*RLAST + m] holds X[i + m*P/RLAST]
+    // registers: X[i + m*P/RLAST] sits at reg[b*RLAST + pos(m)]; the product is
+    // stored back in natural order m, the inverse first pass's input order
+    using DL = DFT<R, PL::RLAST, false>;
+    cpx<R> nat[E];
 #pragma unroll
     for (int b = 0; b < E / PL::RLAST; ++b) {
         const int i = t + b * PL::TPT;
 #pragma unroll
         for (int m = 0; m < PL::RLAST; ++m) {
             const int n = i + m * (P / PL::RLAST);
-            cpx<R> z = reg[b * PL::RLAST + m];
+            cpx<R> z = reg[b * PL::RLAST + DL::pos(m)];
             R hr = Hre[n];
             if (Him) {
                 R hi = conj ? -Him[n] : Him[n];
-                reg[b * PL::RLAST + m] = {z.x * hr - z.y * hi, z.x * hi + z.y * hr};
+                nat[b * PL::RLAST + m] = {z.x * hr - z.y * hi, z.x * hi + z.y * hr};
             } else {
-                reg[b * PL::RLAST + m] = {z.x * hr, z.y * hr};
+                nat[b * PL::RLAST + m] = {z.x * hr, z.y * hr};
             }
         }
     }
+#pragma unroll
+    for (int e = 0; e < E; ++e) reg[e] = nat[e];
     if (PL::RFIRST != PL::RLAST) {
         // hand the spectrum to the inverse transform's first pass through LDS
         __syncthreads();
@@ -467,6 +479,7 @@ __global__ void __launch_bounds__(256)
     Passes<R, P, true, 0>::run(reg, s, tw, t, 1);
 
     // crop to Nu straight to HBM, per-projection max |value|
+    using DI = DFT<R, PL::RLAST, true>;
     R mA = 0, mB = 0;
     R *outA = out + (hasA ? rA : 0) * Nu;
     R *outB = out + (hasB ? rB : 0) * Nu;
@@ -477,7 +490,7 @@ __global__ void __launch_bounds__(256)
         for (int m = 0; m < PL::RLAST; ++m) {
             const int n = i + m * (P / PL::RLAST);
             if (n < Nu) {
-                cpx<R> z = reg[b * PL::RLAST + m];
+                cpx<R> z = reg[b * PL::RLAST + DI::pos(m)];
                 if (hasA) outA[n] = z.x;
                 if (hasB) outB[n] = z.y;
                 R ax = z.x < 0 ? -z.x : z.x, ay = z.y < 0 ? -z.y : z.y;
