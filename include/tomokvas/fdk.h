@@ -129,17 +129,19 @@ int fdk_weight(fdk_ctx *ctx, int dtype, void *d_sino, int nang, void *stream);
  * uint32 = float bits) receives the per-projection max |filtered value|. */
 int fdk_filter(fdk_ctx *ctx, int dtype, void *d_sino, int nang, int weight, int adjoint,
                uint32_t *d_maxabs, void *stream);
-/* bytes of the cell stack for nang projections in storage `cell_dtype` */
-size_t fdk_cells_bytes(fdk_ctx *ctx, int cell_dtype, int nang);
-/* repack filtered [Nv][nang][Nu] (F32 or F64) into the backprojector's cell
- * stack; FP16 uses scale = maxabs/16384 per projection (d_maxabs required) */
+/* bytes of the stored stack for nang projections in storage `store_dtype`:
+ * [nang][Nv][pitch] samples, pitch >= Nu + 68 (zero-padded rows) */
+size_t fdk_stack_bytes(fdk_ctx *ctx, int store_dtype, int nang);
+/* repack filtered [Nv][nang][Nu] (F32 or F64) into the backprojector's stored
+ * stack; FP16 stores value/scale with scale = maxabs/8192 per projection
+ * (d_maxabs required), F32/BF16/F64 store the values (scale 1) */
 int fdk_pack(fdk_ctx *ctx, int dtype_in, const void *d_filtered, int nang,
-             const uint32_t *d_maxabs, int cell_dtype, void *d_cells, void *d_scales,
+             const uint32_t *d_maxabs, int store_dtype, void *d_stack, void *d_scales,
              void *stream);
 /* FBPConeKernel (kernel.cu:68-177) over the whole volume for angles
  * [ang0, ang0+nang): vol (+)= delta_f/p * sum U^2 * bilinear(Q).  vol is F64 for
- * F64 cells, F32 otherwise. */
-int fdk_backproject(fdk_ctx *ctx, int cell_dtype, const void *d_cells,
+ * F64 storage, F32 otherwise. */
+int fdk_backproject(fdk_ctx *ctx, int store_dtype, const void *d_stack,
                     const void *d_scales, int ang0, int nang, void *d_vol, int accumulate,
                     void *stream);
 /* Adjoint of fdk_backproject: voxel-driven transposed-bilinear forward projector

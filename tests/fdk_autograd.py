@@ -27,16 +27,16 @@ class FDKFunction(torch.autograd.Function):
         y = proj.detach().clone()
         st.filter(y, weight=True)
         if f64:
-            cells, scales = st.pack(y, _lib.F64)
-            vol = st.backproject(cells, _lib.F64, scales)
+            stack, scales = st.pack(y, _lib.F64)
+            vol = st.backproject(stack, _lib.F64, scales)
         else:
             maxabs = None
             if storage == _lib.F16:
                 maxabs = torch.from_numpy(
                     y.abs().amax(dim=(0, 2)).cpu().numpy().astype(np.float32).view(np.int32)).cuda()
-            cells, scales = st.pack(y, storage, maxabs)
-            vol = st.backproject(cells, storage, scales)
-        del y, cells
+            stack, scales = st.pack(y, storage, maxabs)
+            vol = st.backproject(stack, storage, scales)
+        del y, stack
         ctx.st = st
         ctx.chunk = chunk
         ctx.taps = taps.detach().cpu().numpy().copy()

@@ -1,0 +1,140 @@
+"""CPU check of the backprojector's LDS footprint boxes (bp_kernel in
+tomokvas_amd/csrc/fdk_backproject.hip).
+
+The kernel stages, per block of 16x16 columns x 32 slices and per angle, the
+detector box its corner columns predict (u and U are linear-fractional in
+(aX, aY), hence extreme at the corners), widened by 2 samples, aligned to 4
+columns and clipped to the detector, then gathers every bilinear tap from that
+box.  This test re-evaluates the box rule in float32 and checks, for every
+active lane and slice of every block and angle, that both taps' columns lie in
+[0, bw] with bw <= 62 and both rows in [0, bh]; and that a box reported empty
+has no tap on the detector.  Geometries: the unit-test scans and the workloads
+(the large ones on a subsample of blocks and angles).
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from tomokvas_amd import geometry
+
+from gpu_util import ogeom, scan
+
+f32 = np.float32
+def fma(a, b, c):  # float32 fma via float64 (exact product)
+    return (np.float64(a) * np.float64(b) + np.float64(c)).astype(f32)
+
+BOX_W, ALIGN = 64, 4
+# rows of the LDS box per z-run length (box_rows in the kernel)
+BOX_ROWS = {16: 64, 24: 80, 32: 96}
+
+
+def footprint(c, s, x0, x1, y0, y1, azlo, azhi, cX, cZ, D, Nu, Nv, box_h):
+    us, Us = [], []
+    ok = True
+    for aX in (x0, x1):
+        for aY in (y0, y1):
+            S = fma(aX, c, f32(aY * s)); T = fma(aY, c, f32(-(aX * s)))
+            den = f32(D - S); ok = ok and den > f32(0.01) * D
+            U = f32(D / den); u = fma(T, U, cX)
+            us.append(u); Us.append(U)
+    if not ok: return (0, 0, -1, 0)
+    umin, umax, Umin, Umax = min(us), max(us), min(Us), max(Us)
+    p = [f32(azlo * Umin), f32(azlo * Umax), f32(azhi * Umin), f32(azhi * Umax)]
+    vlo = f32(min(p) + cZ); vhi = f32(max(p) + cZ)
+    cl = lambda x, n: np.floor(min(max(x, -8.0), n + 8.0))
+    bu0 = max(int(cl(umin, Nu)) - 2, 0) & ~(ALIGN - 1)
+    bu1 = min(int(cl(umax, Nu)) + 3, Nu - 1)
+    bv0 = max(int(cl(vlo, Nv)) - 2, 0)
+    bv1 = min(int(cl(vhi, Nv)) + 3, Nv - 1)
+    bw, bh = bu1 - bu0 + 1, bv1 - bv0 + 1
+    if bw <= 0 or bh <= 0: return (0, 0, 0, 0)
+    if bw >= BOX_W or bh >= box_h: return (0, 0, -1, 0)
+    return (bu0, bv0, bw, bh)
+
+def check(g, ZR=32, tile_stride=1, ang_stride=1):
+    box_h = BOX_ROWS[ZR]
+    """Counts of staged / empty / HBM-gathered (block, angle) pairs and taps."""
+    ax, ay, az = [a.astype(f32) for a in O.axis_tables(g)]
+    nz_pad = ((g.Nz + 63) // 64) * 64 + 64
+    # extend az linearly as the host does
+    if len(az) < nz_pad:
+        d = az[1] - az[0] if len(az) > 1 else f32(1)
+        az = np.concatenate([az, (az[-1] + d * np.arange(1, nz_pad - len(az) + 1)).astype(f32)])
+    s64, c64 = O.sincos(g)
+    cs, ss = c64.astype(f32), s64.astype(f32)
+    cX, cZ, D = f32(g.cX), f32(g.cZ), f32(g.D)
+    Nu, Nv = g.Nu, g.Nv
+    umaxv, vmaxv = f32(Nu - 1), f32(Nv - 1)
+    maskR2 = 0.25 * Nu * Nu
+    tiles_x, tiles_y = (g.Nx + 15) // 16, (g.Ny + 15) // 16
+    nzr = (g.Nz + ZR - 1) // ZR
+    stats = dict(staged=0, empty=0, hbm=0, taps=0, maxbw=0, maxbh=0)
+    for zr in range(nzr):
+        azr = az[zr * ZR: zr * ZR + ZR]
+        azlo, azhi = min(azr[0], azr[-1]), max(azr[0], azr[-1])
+        for ty in range(0, tiles_y, tile_stride):
+            for tx in range(0, tiles_x, tile_stride):
+                xs = np.arange(tx * 16, min(tx * 16 + 16, g.Nx)); ys = np.arange(ty * 16, min(ty * 16 + 16, g.Ny))
+                X, Y = np.meshgrid(ax[xs], ay[ys]); X = X.ravel(); Y = Y.ravel()
+                inside = X.astype(np.float64) ** 2 + Y.astype(np.float64) ** 2 < maskR2
+                if not inside.any(): continue
+                x0, x1 = ax[tx * 16], ax[min(tx * 16 + 15, g.Nx - 1)]
+                y0, y1 = ay[ty * 16], ay[min(ty * 16 + 15, g.Ny - 1)]
+                for b in range(0, g.Nang, ang_stride):
+                    bu0, bv0, bw, bh = footprint(cs[b], ss[b], x0, x1, y0, y1, azlo, azhi, cX, cZ, D, Nu, Nv, box_h)
+                    if bw == 0:
+                        # nothing may land on the detector
+                        kind = 'empty'
+                    elif bw < 0:
+                        stats['hbm'] += 1; continue
+                    else:
+                        kind = 'staged'
+                        stats['maxbw'] = max(stats['maxbw'], bw); stats['maxbh'] = max(stats['maxbh'], bh)
+                    c, s = cs[b], ss[b]
+                    S = fma(X, c, (Y * s).astype(f32)); T = fma(Y, c, (-(X * s)).astype(f32))
+                    U = (D / (D - S)).astype(f32); u = fma(T, U, cX)
+                    act = inside & (u >= 0) & (u <= umaxv)
+                    if kind == 'empty':
+                        if act.any():
+                            v = fma(azr[None, :], U[act, None], cZ)
+                            assert not ((v >= 0) & (v <= vmaxv)).any(), ("empty box but a tap lands", zr, tx, ty, b)
+                        stats['empty'] += 1; continue
+                    stats['staged'] += 1
+                    if not act.any(): continue
+                    iu = u[act].astype(np.int64)
+                    v = fma(azr[None, :], U[act, None], cZ)
+                    ok = (v >= 0) & (v <= vmaxv)
+                    iv = np.floor(v).astype(np.int64)
+                    col = iu - bu0
+                    assert col.min() >= 0 and col.max() + 1 <= bw and bw <= 62, ("col", zr, tx, ty, b, col.min(), col.max(), bw)
+                    rows = (iv - bv0)[ok]
+                    if rows.size:
+                        assert rows.min() >= 0 and rows.max() + 1 <= bh, ("row", zr, tx, ty, b, rows.min(), rows.max(), bh)
+                    stats['taps'] += int(ok.sum())
+    return stats
+
+
+CASES = {
+    "odd": (lambda: ogeom(scan(37, 23, 7, 19, 23, 17, D=60.0, tau=0.1, scale=1.3, cX=17.2,
+                               cZ=11.7, rot=0.3, flips=(1, 0, 1))), 1, 1),
+    "small": (lambda: ogeom(scan(32, 32, 24, 32, 32, 32, D=100.0, tau=0.2)), 1, 1),
+    "near_source": (lambda: ogeom(scan(64, 48, 30, 48, 48, 40, D=40.0, tau=0.1, scale=1.0)), 1, 1),
+    "wide": (lambda: ogeom(scan(96, 96, 20, 40, 40, 40, D=300.0, tau=0.1, scale=4.0)), 1, 1),
+    "cfg1": (lambda: ogeom(geometry.workloads()["cfg1"].geom), 2, 7),
+    "cfg3": (lambda: ogeom(geometry.workloads()["cfg3"].geom), 23, 197),
+    "cfg5": (lambda: ogeom(geometry.workloads()["cfg5"].geom), 23, 197),
+}
+
+
+@pytest.mark.parametrize("zr", sorted(BOX_ROWS))
+@pytest.mark.parametrize("name", list(CASES))
+def test_footprint_boxes_cover_every_tap(name, zr):
+    make, ts, an = CASES[name]
+    st = check(make(), ZR=zr, tile_stride=ts, ang_stride=an)
+    assert st["taps"] > 0
+    if name.startswith("cfg"):
+        # the workloads never fall back to HBM gathers
+        assert st["hbm"] == 0, st
+    if name == "wide":
+        # this geometry exercises the HBM-gather pass next to the staged one
+        assert st["hbm"] > 0 and st["staged"] > 0, st

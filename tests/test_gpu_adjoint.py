@@ -31,8 +31,8 @@ def test_dot_product_adjoint_fp64(torch, geo):
     gen = torch.Generator(device="cuda").manual_seed(1)
     y = torch.randn(st.sino_shape(), dtype=torch.float64, device="cuda", generator=gen)
     x = torch.randn(st.vol_shape, dtype=torch.float64, device="cuda", generator=gen)
-    cells, scales = st.pack(y, _lib.F64)
-    BPy = st.backproject(cells, _lib.F64, scales)
+    stack, scales = st.pack(y, _lib.F64)
+    BPy = st.backproject(stack, _lib.F64, scales)
     FPx = st.forward_project(x)
     lhs = float((BPy * x).sum())
     rhs = float((y * FPx).sum())

@@ -54,7 +54,7 @@ _SIGS = {
     "fdk_prepare": (_i, [_vp]),
     "fdk_weight": (_i, [_vp, _i, _vp, _i, _vp]),
     "fdk_filter": (_i, [_vp, _i, _vp, _i, _i, _i, _vp, _vp]),
-    "fdk_cells_bytes": (C.c_size_t, [_vp, _i, _i]),
+    "fdk_stack_bytes": (C.c_size_t, [_vp, _i, _i]),
     "fdk_pack": (_i, [_vp, _i, _vp, _i, _vp, _i, _vp, _vp, _vp]),
     "fdk_backproject": (_i, [_vp, _i, _vp, _vp, _i, _i, _vp, _i, _vp]),
     "fdk_forward_project": (_i, [_vp, _i, _vp, _i, _i, _vp, _vp]),

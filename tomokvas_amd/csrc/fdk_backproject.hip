@@ -7,135 +7,155 @@
 //   samples outside [0,Nu-1] x [0,Nv-1] contribute 0.
 //
 // MI355X design:
-//  * filtered projections are stored as "quad cells": for every detector
-//    sample (v,u) the four bilinear coefficients
-//        A = Q(v,u)              B = Q(v+1,u) - Q(v,u)
-//        C = Q(v,u+1) - Q(v,u)   D = Q(v+1,u+1) - Q(v+1,u) - Q(v,u+1) + Q(v,u)
-//    so bilinear(v+fv, u+fu) = (A + fu C) + fv (B + fu D): ONE aligned 16-byte
-//    (FP32) or 8-byte (FP16) load and three FMAs per voxel-update;
-//  * a thread owns one (x,y) column and a run of ZR consecutive slices in
-//    registers: S, T, U, u = T*U + cX, fu and U^2 depend only on (x, y, beta)
-//    and are paid once per ZR updates;
-//  * per update: v = aZ*U + cZ, row = trunc(v) (v >= 0 there), fv = fract(v),
-//    one address, one load, 4 FMAs;
-//  * a wave covers an 8x8 column tile (compact detector footprint), a
-//    workgroup 16x16 columns; all lanes walk the angles in lockstep, so the
-//    per-angle cos/sin/scale and the projection base address are scalar;
-//  * z-runs that project entirely inside the detector for every lane of the
-//    wave (the common case) skip the per-slice range test;
+//  * a workgroup owns a block of 16x16 columns x ZR slices; a thread owns one
+//    (x,y) column and its ZR slices in registers: S, T, U, u = T*U + cX and U^2
+//    depend only on (x, y, beta) and are paid once per ZR updates;
+//  * per angle, the block's detector footprint -- a box of samples that the
+//    workgroup derives from its corner columns (u and U are linear-fractional
+//    in (aX, aY), so their extremes over the block sit at the corners) -- is
+//    staged from HBM into LDS with coalesced loads and converted to FP32;
+//    every bilinear gather of the angle then reads LDS, so L2/HBM carry about
+//    one stored sample per two voxel-updates instead of four taps per update;
+//  * the stored stack is plain and angle-major ([beta][v][pitch]) in the
+//    storage type: FP32, FP16 with a per-projection scale, or BF16; FP16 and
+//    BF16 halve the staging bytes; interpolation and accumulation are FP32;
+//  * z-runs that project entirely inside the detector for every active lane of
+//    a wave (the common case) skip the per-slice range test;
 //  * workgroup ids are remapped so the workgroups an XCD runs together cover a
-//    compact block of the volume and share detector rows in that XCD's L2;
-//  * FP16 quads feed v_fma_mix_f32 directly: FP16 coefficients, FP32 weights
-//    and FP32 accumulators, half the gather bytes of FP32 quads.
+//    compact part of the volume and share detector rows in that XCD's L2;
+//  * boxes larger than the LDS tile (extreme magnification, blocks near the
+//    source) and the FP64 instantiation gather straight from HBM with the same
+//    arithmetic.
 #include "fdk_internal.hpp"
 
 #include <hip/hip_bf16.h>
 #include <hip/hip_fp16.h>
 
-// Tuning knobs (defaults chosen by measurement on MI355X):
-//   TKV_BP_ZR            slices per thread (register z-run) for FP32/FP16/BF16
-//   TKV_BP_LB            gathers per lane issued together
-//   TKV_BP_BATCH_BARRIER 1 = no load is scheduled ahead of the previous batch
-#ifndef TKV_BP_ZR
-#define TKV_BP_ZR 32
+#include <type_traits>
+
+// Tuning knobs (defaults chosen by measurement on MI355X, 1024^3 from
+// 1800 x 2048^2):
+//   TKV_BP_ZR_F32 / _F16 / _BF16   slices per thread (register z-run) per
+//                                  storage type (FP64 uses 8)
+//   TKV_BP_LB                      LDS gathers per lane issued together
+//   TKV_BP_BOXP                    LDS row pitch of the staged box in floats
+//                                  (>= 64, multiple of 4)
+//   TKV_BP_F16_PAIRS               FP16 storage keeps FP16 sample pairs in LDS
+//                                  (one 32-bit LDS word per tap pair, lerp
+//                                  along v in packed FP16); 0: FP32 box
+#ifndef TKV_BP_ZR_F32
+#define TKV_BP_ZR_F32 16
+#endif
+#ifndef TKV_BP_ZR_F16
+#define TKV_BP_ZR_F16 32
+#endif
+#ifndef TKV_BP_ZR_BF16
+#define TKV_BP_ZR_BF16 24
 #endif
 #ifndef TKV_BP_LB
-#define TKV_BP_LB 8
+#define TKV_BP_LB 4
 #endif
-#ifndef TKV_BP_BATCH_BARRIER
-#define TKV_BP_BATCH_BARRIER 1
+#ifndef TKV_BP_BOXP
+#define TKV_BP_BOXP 64
+#endif
+#ifndef TKV_BP_F16_PAIRS
+#define TKV_BP_F16_PAIRS 1
 #endif
 
 namespace tkv {
 
+// staged box: at most BOX_W - 1 columns (+1 neighbour) and box_rows<ZR>() - 1
+// rows (+1); the row budget covers a magnification of about 1.4 at two
+// detector samples per voxel
+constexpr int BOX_W = 64;
+constexpr int BOX_P = TKV_BP_BOXP;
+template <int ZR>
+constexpr int box_rows() { return ZR > 24 ? 96 : ZR > 16 ? 80 : 64; }
+static_assert(box_rows<64>() <= STACK_TAIL_ROWS, "stored stack tail shorter than a box");
+constexpr int BOX_ALIGN = 4;   // box columns start on a multiple of 4 samples
+static_assert(BOX_P >= BOX_W && BOX_P % 4 == 0, "box pitch");
+
+// Row pitch of the stored stack in samples: a multiple of 64 with at least
+// BOX_W + 4 zero samples after column Nu-1, so a staged box row that starts
+// inside the detector never leaves its own row.
+int stack_pitch(int Nu) { return ((Nu + BOX_W + 4 + 63) / 64) * 64; }
+
 template <typename R>
 struct BPArgs {
-    const char *cells;
-    size_t angle_bytes;           // bytes per projection in the cell stack
-    uint32_t row_bytes;           // bytes per detector row of cells
+    const char *stack;
+    size_t angle_bytes;           // bytes per projection of the stack
+    uint32_t row_bytes;           // bytes per detector row of the stack
     const R *ang;                 // (cos, sin) per angle
     const R *scales;              // per angle dequantisation factor
     const R *ax, *ay, *az;
     R *vol;
-    int Nx, Ny, Nz, nang;
+    int Nx, Ny, Nz, Nu, Nv, nang;
     R cX, cZ, D, umax, vmax, out_scale;
     double maskR2;
     int accumulate;
-    int tiles_x, tiles_xy, nzrun, nblocks;
+    int tiles_x, tiles_xy, nblocks;
 };
 
-// ---- quad cells: load one, evaluate (A + fu C) + fv (B + fu D) ------------------------
-struct quad_f32;
-struct quad_f16;
-struct quad_bf16;
-struct quad_f64;
-
-template <typename Q>
-struct Quad;
-
+// ---- storage types ----------------------------------------------------------------
+struct st_f32;
+struct st_f16;
+struct st_bf16;
+struct st_f64;
+template <typename S>
+struct Store;
 template <>
-struct Quad<quad_f32> {
+struct Store<st_f32> {
     using R = float;
-    using L = float4;   // (A, B, C, D)
-    static constexpr int BYTES = 16;
-    static __device__ __forceinline__ float eval(const L &q, float fu, float fv)
+    static constexpr int BYTES = 4;
+    static constexpr bool STAGED = true;
+    static constexpr int ZR = TKV_BP_ZR_F32;
+    static constexpr bool PAIRS = false;
+    static __device__ __forceinline__ float get(const char *p) { return *(const float *)p; }
+    static __device__ __forceinline__ float half(unsigned, int) { return 0.f; }
+};
+template <>
+struct Store<st_f16> {
+    using R = float;
+    static constexpr int BYTES = 2;
+    static constexpr bool STAGED = true;
+    static constexpr int ZR = TKV_BP_ZR_F16;
+    static constexpr bool PAIRS = TKV_BP_F16_PAIRS != 0;
+    static __device__ __forceinline__ float get(const char *p)
     {
-        float t0 = fmaf(fu, q.z, q.x);
-        float t1 = fmaf(fu, q.w, q.y);
-        return fmaf(fv, t1, t0);
+        return __half2float(__ushort_as_half(*(const unsigned short *)p));
+    }
+    static __device__ __forceinline__ float half(unsigned w, int hi)
+    {
+        return __half2float(__ushort_as_half((unsigned short)(hi ? w >> 16 : w & 0xffffu)));
     }
 };
-
 template <>
-struct Quad<quad_f64> {
+struct Store<st_bf16> {
+    using R = float;
+    static constexpr int BYTES = 2;
+    static constexpr bool STAGED = true;
+    static constexpr int ZR = TKV_BP_ZR_BF16;
+    static constexpr bool PAIRS = false;
+    static __device__ __forceinline__ float get(const char *p)
+    {
+        return __uint_as_float((unsigned)(*(const unsigned short *)p) << 16);
+    }
+    static __device__ __forceinline__ float half(unsigned w, int hi)
+    {
+        return __uint_as_float(hi ? (w & 0xffff0000u) : (w << 16));
+    }
+};
+template <>
+struct Store<st_f64> {
     using R = double;
-    using L = double4;
-    static constexpr int BYTES = 32;
-    static __device__ __forceinline__ double eval(const L &q, double fu, double fv)
-    {
-        double t0 = fma(fu, q.z, q.x);
-        double t1 = fma(fu, q.w, q.y);
-        return fma(fv, t1, t0);
-    }
-};
-
-template <>
-struct Quad<quad_f16> {
-    using R = float;
-    using L = uint2;    // x = (A, B) halves, y = (C, D) halves
     static constexpr int BYTES = 8;
-    // FP16 operands fold into v_fma_mix_f32: FP32 arithmetic, no conversions
-    static __device__ __forceinline__ float eval(const L &q, float fu, float fv)
-    {
-        __half2 ab = *reinterpret_cast<const __half2 *>(&q.x);
-        __half2 cd = *reinterpret_cast<const __half2 *>(&q.y);
-        float t0 = fmaf(fu, __low2float(cd), __low2float(ab));
-        float t1 = fmaf(fu, __high2float(cd), __high2float(ab));
-        return fmaf(fv, t1, t0);
-    }
+    static constexpr bool STAGED = false;
+    static constexpr int ZR = 8;
+    static constexpr bool PAIRS = false;
+    static __device__ __forceinline__ double get(const char *p) { return *(const double *)p; }
+    static __device__ __forceinline__ float half(unsigned, int) { return 0.f; }
 };
 
-template <>
-struct Quad<quad_bf16> {
-    using R = float;
-    using L = uint2;
-    static constexpr int BYTES = 8;
-    static __device__ __forceinline__ float eval(const L &q, float fu, float fv)
-    {
-        float A = __uint_as_float(q.x << 16), B = __uint_as_float(q.x & 0xffff0000u);
-        float C = __uint_as_float(q.y << 16), D = __uint_as_float(q.y & 0xffff0000u);
-        float t0 = fmaf(fu, C, A);
-        float t1 = fmaf(fu, D, B);
-        return fmaf(fv, t1, t0);
-    }
-};
-
-template <typename R>
-__device__ __forceinline__ R rfloor(R x);
-template <>
-__device__ __forceinline__ float rfloor(float x) { return floorf(x); }
-template <>
-__device__ __forceinline__ double rfloor(double x) { return floor(x); }
 template <typename R>
 __device__ __forceinline__ R rfma(R a, R b, R c);
 template <>
@@ -150,18 +170,221 @@ __device__ __forceinline__ float rfract(float x) { return __builtin_amdgcn_fract
 template <>
 __device__ __forceinline__ double rfract(double x) { return __builtin_amdgcn_fract(x); }
 
-template <typename Q, typename L>
-__device__ __forceinline__ L load_quad(const char *base, uint32_t off)
+// ---- footprint box of a block for one angle ----------------------------------------
+// Returns (bu0, bv0, bw, bh): the detector samples [bu0, bu0+bw) x [bv0, bv0+bh)
+// that the block's bilinear taps can touch, widened by 2 samples on each side
+// for the rounding of the per-column values (the columns evaluate the same
+// float expressions as the corners); bw == 0: the block misses the detector;
+// bw < 0: gather from HBM (box beyond the LDS tile, or a block corner near or
+// behind the source plane, where U is not monotone over the block).
+__device__ int4 footprint(float c, float s, float x0, float x1, float y0, float y1, float azlo,
+                          float azhi, float cX, float cZ, float D, int Nu, int Nv, int box_h)
 {
-    return *reinterpret_cast<const L *>(base + off);
+    float umin = 3.0e38f, umax = -3.0e38f, Umin = 3.0e38f, Umax = -3.0e38f;
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const float aX = (k & 1) ? x1 : x0, aY = (k & 2) ? y1 : y0;
+        const float S = fmaf(aX, c, aY * s);
+        const float T = fmaf(aY, c, -(aX * s));
+        const float den = D - S;
+        ok = ok && den > 0.01f * D;
+        const float U = D / den;
+        const float u = fmaf(T, U, cX);
+        umin = fminf(umin, u);
+        umax = fmaxf(umax, u);
+        Umin = fminf(Umin, U);
+        Umax = fmaxf(Umax, U);
+    }
+    if (!ok) return make_int4(0, 0, -1, 0);
+    const float p0 = azlo * Umin, p1 = azlo * Umax, p2 = azhi * Umin, p3 = azhi * Umax;
+    const float vlo = fminf(fminf(p0, p1), fminf(p2, p3)) + cZ;
+    const float vhi = fmaxf(fmaxf(p0, p1), fmaxf(p2, p3)) + cZ;
+    // clamp before converting; anything beyond the detector by 4 samples is empty
+    const float fu0 = floorf(fminf(fmaxf(umin, -8.0f), (float)Nu + 8.0f));
+    const float fu1 = floorf(fminf(fmaxf(umax, -8.0f), (float)Nu + 8.0f));
+    const float fv0 = floorf(fminf(fmaxf(vlo, -8.0f), (float)Nv + 8.0f));
+    const float fv1 = floorf(fminf(fmaxf(vhi, -8.0f), (float)Nv + 8.0f));
+    const int bu0 = max((int)fu0 - 2, 0) & ~(BOX_ALIGN - 1);
+    const int bu1 = min((int)fu1 + 3, Nu - 1);
+    const int bv0 = max((int)fv0 - 2, 0);
+    const int bv1 = min((int)fv1 + 3, Nv - 1);
+    const int bw = bu1 - bu0 + 1, bh = bv1 - bv0 + 1;
+    if (bw <= 0 || bh <= 0) return make_int4(0, 0, 0, 0);
+    if (bw >= BOX_W || bh >= box_h) return make_int4(0, 0, -1, 0);
+    return make_int4(bu0, bv0, bw, bh);
 }
 
-template <typename Q, int ZR, int LB>
-__global__ void __launch_bounds__(256) bp_kernel(BPArgs<typename Quad<Q>::R> a)
+// ---- LDS gathers of one angle for LB slices of a thread's z-run ----------------------
+// acc[k] += W * bilinear(box; u, v_k) with v_k = az[k]*Uv + cz; colb is the
+// byte offset of the thread's column minus that of the box's first row.
+// FAST: every v_k of every lane of the wave lies inside [0.01, Nv-1.01].
+template <bool FAST, int LB>
+__device__ __forceinline__ void gather_lds(float *acc, const float *az, const float *box,
+                                           float Uv, float cz, int colb, float wu0, float wu1,
+                                           float vmax)
 {
-    using R = typename Quad<Q>::R;
-    using QT = Quad<Q>;
-    using L = typename QT::L;
+    const char *lds = (const char *)box;
+    float c0[LB], c1[LB];
+    bool okk[LB];
+#pragma unroll
+    for (int k = 0; k < LB; ++k) {
+        const float v = fmaf(az[k], Uv, cz);
+        const float fv = __builtin_amdgcn_fractf(v);
+        bool ok = true;
+        int off = (int)v * (4 * BOX_P) + colb;
+        if (!FAST) {
+            ok = v >= 0.f && v <= vmax;
+            off = ok ? off : 0;
+        }
+        const float *p0 = (const float *)(lds + off);
+        const float q00 = p0[0], q01 = p0[1];
+        const float q10 = p0[BOX_P], q11 = p0[BOX_P + 1];
+        c0[k] = fmaf(fv, q10 - q00, q00);
+        c1[k] = fmaf(fv, q11 - q01, q01);
+        okk[k] = ok;
+    }
+    // taps outside the detector get zero weight (the staged values are finite)
+#pragma unroll
+    for (int k = 0; k < LB; ++k)
+        acc[k] = fmaf(okk[k] ? wu1 : 0.f, c1[k], fmaf(okk[k] ? wu0 : 0.f, c0[k], acc[k]));
+}
+
+// FP16 pairs in LDS: word (r, c) holds samples c and c+1 of row r; one
+// ds_read2 fetches both rows of the 2x2 neighbourhood; the lerp along v runs
+// in packed FP16, the weights and the sum in FP32.
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+template <bool FAST, int LB>
+__device__ __forceinline__ void gather_pairs(float *acc, const float *az, const float *box,
+                                             float Uv, float cz, int colb, float wu0,
+                                             float wu1, float vmax)
+{
+    const char *lds = (const char *)box;
+    half2_t c[LB];
+    bool okk[LB];
+#pragma unroll
+    for (int k = 0; k < LB; ++k) {
+        const float v = fmaf(az[k], Uv, cz);
+        const float fv = __builtin_amdgcn_fractf(v);
+        bool ok = true;
+        int off = (int)v * (4 * BOX_P) + colb;
+        if (!FAST) {
+            ok = v >= 0.f && v <= vmax;
+            off = ok ? off : 0;
+        }
+        const half2_t w0 = *(const half2_t *)(lds + off);
+        const half2_t w1 = *(const half2_t *)(lds + off + 4 * BOX_P);
+        const half2_t f = __builtin_bit_cast(half2_t, __builtin_amdgcn_cvt_pkrtz(fv, fv));
+        c[k] = f * (w1 - w0) + w0;
+        okk[k] = ok;
+    }
+#pragma unroll
+    for (int k = 0; k < LB; ++k)
+        acc[k] = fmaf(okk[k] ? wu1 : 0.f, (float)c[k].y,
+                      fmaf(okk[k] ? wu0 : 0.f, (float)c[k].x, acc[k]));
+}
+
+// all ZR slices of a thread for one staged angle, LB gathers at a time
+template <bool FAST, bool PAIRS, int ZR, int LB>
+__device__ __forceinline__ void gather_run(float (&acc)[ZR], const float (&az)[ZR],
+                                           const float *box, float Uv, float cz, int colb,
+                                           float wu0, float wu1, float vmax)
+{
+#pragma unroll
+    for (int k0 = 0; k0 < ZR; k0 += LB) {
+        if constexpr (PAIRS)
+            gather_pairs<FAST, LB>(acc + k0, az + k0, box, Uv, cz, colb, wu0, wu1, vmax);
+        else
+            gather_lds<FAST, LB>(acc + k0, az + k0, box, Uv, cz, colb, wu0, wu1, vmax);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+// Box staging in two steps, so the loads of the next angle's box fly while
+// the current angle is gathered: issue() loads rows [bv0, bv0+bh] of a
+// projection (the last is the +1 neighbour row; rows past the detector are
+// zero) x columns [bu0, bu0+64) into registers, 4 samples per lane and row,
+// 16 lanes per row (columns past the row end read the stack's zero padding);
+// commit() writes them into the LDS box.  FP32 boxes hold the samples as
+// floats; FP16 pair boxes hold words (q[c], q[c+1]), the 4th word of a lane
+// taking the next lane's first sample (word 63 is never read).
+template <typename S, int ROWS>
+struct BoxStage {
+    using ST = Store<S>;
+    static constexpr int PER = (ROWS + 15) / 16;   // rows per thread
+    using Raw = typename std::conditional<ST::BYTES == 4, float4, uint2>::type;
+    Raw raw[PER];
+
+    __device__ __forceinline__ void issue(const char *proj, int bu0, int bv0, int bh,
+                                          uint32_t row_bytes, int Nv)
+    {
+        // buffer loads: a scalar descriptor at the box origin whose range ends
+        // with the last row inside the detector, so rows past it read as zero
+        // (raw buffer range check), and one 32-bit offset per row
+        const char *base = proj + (size_t)bv0 * row_bytes + (size_t)bu0 * ST::BYTES;
+        const int rows = min(bh, Nv - 1 - bv0);   // last row inside the detector
+        const int nrec = (rows + 1) * (int)row_bytes - bu0 * ST::BYTES;
+        const __amdgpu_buffer_rsrc_t rsrc =
+            __builtin_amdgcn_make_buffer_rsrc((void *)base, (short)0, nrec, 0x00020000);
+        const uint32_t off0 = (threadIdx.x >> 4) * row_bytes + (threadIdx.x & 15) * 4 * ST::BYTES;
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int off = (int)(off0 + (uint32_t)(16 * i) * row_bytes);
+            if constexpr (ST::BYTES == 4) {
+                const auto v = __builtin_amdgcn_raw_buffer_load_b128(rsrc, off, 0, 0);
+                raw[i] = make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]),
+                                     __uint_as_float(v[2]), __uint_as_float(v[3]));
+            } else {
+                const auto v = __builtin_amdgcn_raw_buffer_load_b64(rsrc, off, 0, 0);
+                raw[i] = make_uint2(v[0], v[1]);
+            }
+        }
+    }
+
+    __device__ __forceinline__ void commit(float *box, int bh) const
+    {
+        const int r0 = threadIdx.x >> 4, col = (threadIdx.x & 15) * 4;
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int r = r0 + 16 * i;
+            float *dst = box + r * BOX_P + col;
+            if constexpr (ST::PAIRS) {
+                const uint2 w = raw[i];
+                const unsigned nx = __shfl_down(w.x, 1, 16);
+                if (r <= bh)
+                    *(uint4 *)dst = make_uint4(w.x, (w.x >> 16) | (w.y << 16), w.y,
+                                               (w.y >> 16) | (nx << 16));
+            } else if constexpr (ST::BYTES == 4) {
+                if (r <= bh) *(float4 *)dst = raw[i];
+            } else {
+                const uint2 w = raw[i];
+                if (r <= bh)
+                    *(float4 *)dst = make_float4(ST::half(w.x, 0), ST::half(w.x, 1),
+                                                 ST::half(w.y, 0), ST::half(w.y, 1));
+            }
+        }
+    }
+};
+
+// a workgroup-uniform value, kept in a scalar register
+template <typename R>
+__device__ __forceinline__ R uniform_load(const R *p) { return *p; }
+template <>
+__device__ __forceinline__ float uniform_load(const float *p)
+{
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(*p)));
+}
+
+// ---- the backprojector --------------------------------------------------------------
+template <typename S, int ZR, int LB>
+__global__ void __launch_bounds__(256) bp_kernel(BPArgs<typename Store<S>::R> a)
+{
+    using R = typename Store<S>::R;
+    using ST = Store<S>;
+    constexpr bool STAGED = ST::STAGED;
+    constexpr int BOX_H = box_rows<ZR>();
+    __shared__ float box[STAGED ? BOX_P * BOX_H : 1];
+    __shared__ int4 boxes[STAGED ? 128 : 1];   // footprints of two groups of 64 angles
 
     // XCD-aware block order: hardware dispatches workgroup i to XCD i % 8; the
     // remap gives each XCD a contiguous range of logical blocks.
@@ -170,8 +393,8 @@ __global__ void __launch_bounds__(256) bp_kernel(BPArgs<typename Quad<Q>::R> a)
     const int per = (nb + 7) / 8;
     const int xcd = bid & 7;
     const int full = nb - (per - 1) * 8;   // XCDs that receive `per` blocks
-    int lb = (xcd < full) ? xcd * per + (bid >> 3)
-                          : full * per + (xcd - full) * (per - 1) + (bid >> 3);
+    const int lb = (xcd < full) ? xcd * per + (bid >> 3)
+                                : full * per + (xcd - full) * (per - 1) + (bid >> 3);
     const int zr = lb / a.tiles_xy;        // z-run major: neighbours share rows
     const int tile = lb - zr * a.tiles_xy;
     const int ty = tile / a.tiles_x, tx = tile - ty * a.tiles_x;
@@ -191,74 +414,151 @@ __global__ void __launch_bounds__(256) bp_kernel(BPArgs<typename Quad<Q>::R> a)
 #pragma unroll
     for (int k = 0; k < ZR; ++k) acc[k] = 0;
 
-    if (__any(inside)) {
+    // every wave of the workgroup joins the staging barriers, so the decision to
+    // skip the angle loop is workgroup-wide
+    if (__syncthreads_or(inside)) {
         R az[ZR];
 #pragma unroll
-        for (int k = 0; k < ZR; ++k) az[k] = a.az[z0 + k];
-        const char *proj = a.cells;
-        for (int b = 0; b < a.nang; ++b, proj += a.angle_bytes) {
-            const R c = a.ang[2 * b], s = a.ang[2 * b + 1];
-            const R sc = a.scales[b];
-            // the reference's S, T, U and u with nvcc's contractions
-            R S = rfma(aX, c, aY * s);
-            R T = rfma(aY, c, -(aX * s));
-            R U = a.D / (a.D - S);
-            R u = rfma(T, U, a.cX);
-            const bool act = inside && u >= (R)0 && u <= a.umax;
-            const R fu = rfract(u);
-            const int iu = act ? (int)u : 0;            // u >= 0: trunc = floor
-            const R W = act ? U * U * sc : (R)0;
-            const R Uv = act ? U : (R)0;
-            const R cz = act ? a.cZ : (R)0;
-            const uint32_t colb = (uint32_t)iu * (uint32_t)QT::BYTES;
-            const R v_lo = rfma(az[0], Uv, cz), v_hi = rfma(az[ZR - 1], Uv, cz);
-            const R vmin = v_lo < v_hi ? v_lo : v_hi, vmax = v_lo < v_hi ? v_hi : v_lo;
-            // (a 0.01-row margin keeps float rounding of the slice table away
-            // from the unchecked path's row bounds)
-            const bool fast = __all(vmin >= (R)0.01 && vmax <= a.vmax - (R)0.01);
-#pragma unroll
-            for (int k0 = 0; k0 < ZR; k0 += LB) {
-                R val[LB];
-                if (fast) {
-                    // whole run inside the detector for every lane: no range tests
-                    uint32_t off[LB];
-                    R fv[LB];
-#pragma unroll
-                    for (int k = 0; k < LB; ++k) {
-                        R v = rfma(az[k0 + k], Uv, cz);
-                        fv[k] = rfract(v);
-                        off[k] = __umul24((uint32_t)(int)v, a.row_bytes) + colb;
-                    }
-                    L q[LB];
-#pragma unroll
-                    for (int k = 0; k < LB; ++k) q[k] = load_quad<Q, L>(proj, off[k]);
-#pragma unroll
-                    for (int k = 0; k < LB; ++k) val[k] = QT::eval(q[k], fu, fv[k]);
-                } else {
-                    uint32_t off[LB];
-                    R fv[LB];
-                    bool ok[LB];
-#pragma unroll
-                    for (int k = 0; k < LB; ++k) {
-                        R v = rfma(az[k0 + k], Uv, cz);
-                        ok[k] = v >= (R)0 && v <= a.vmax;
-                        fv[k] = rfract(v);
-                        off[k] = __umul24(ok[k] ? (uint32_t)(int)v : 0u, a.row_bytes) + colb;
-                    }
-                    L q[LB];
-#pragma unroll
-                    for (int k = 0; k < LB; ++k) q[k] = load_quad<Q, L>(proj, off[k]);
-#pragma unroll
-                    for (int k = 0; k < LB; ++k)
-                        val[k] = ok[k] ? QT::eval(q[k], fu, fv[k]) : (R)0;
+        for (int k = 0; k < ZR; ++k) az[k] = uniform_load(a.az + z0 + k);
+        const R azlo = az[0] < az[ZR - 1] ? az[0] : az[ZR - 1];
+        const R azhi = az[0] < az[ZR - 1] ? az[ZR - 1] : az[0];
+        // block corner coordinates
+        const int xa = tx * 16, xb = min(xa + 15, a.Nx - 1);
+        const int ya = ty * 16, yb = min(ya + 15, a.Ny - 1);
+        const float cx0 = (float)a.ax[xa], cx1 = (float)a.ax[xb];
+        const float cy0 = (float)a.ay[ya], cy1 = (float)a.ay[yb];
+
+        // pass 1: angles whose footprint fits the LDS tile
+        bool need_hbm = !STAGED;
+        if constexpr (STAGED) {
+            // footprints of 64 angles, one per lane of wave 0, into slot (g & 1)
+            auto boxes_of_group = [&](int g) {
+                const int bb = g * 64 + lane;
+                if (wave == 0 && bb < a.nang)
+                    boxes[(g & 1) * 64 + lane] =
+                        footprint(a.ang[2 * bb], a.ang[2 * bb + 1], cx0, cx1, cy0, cy1, azlo,
+                                  azhi, a.cX, a.cZ, a.D, a.Nu, a.Nv, BOX_H);
+            };
+            boxes_of_group(0);
+            boxes_of_group(1);
+            __syncthreads();
+            BoxStage<S, BOX_H> stage;
+            auto uniform_box = [&](int bb) {
+                const int4 q = boxes[bb & 127];
+                return make_int4(__builtin_amdgcn_readfirstlane(q.x),
+                                 __builtin_amdgcn_readfirstlane(q.y),
+                                 __builtin_amdgcn_readfirstlane(q.z),
+                                 __builtin_amdgcn_readfirstlane(q.w));
+            };
+            // The prefetch is issued for every angle (an unconditional issue
+            // keeps one register set live): angles without a staged box get an
+            // empty descriptor range (bh = -1), whose loads return zeros.
+            {
+                const int4 b0 = uniform_box(0);
+                stage.issue(a.stack, b0.z > 0 ? b0.x : 0, b0.z > 0 ? b0.y : 0,
+                            b0.z > 0 ? b0.w : -1, a.row_bytes, a.Nv);
+            }
+            const char *proj = a.stack;
+            for (int b = 0; b < a.nang; ++b, proj += a.angle_bytes) {
+                if ((b & 63) == 0 && b > 0) {
+                    // the slot of group b/64 - 1 is free once every wave is here
+                    __syncthreads();
+                    boxes_of_group((b >> 6) + 1);
+                    __syncthreads();
                 }
+                const int4 bx = uniform_box(b);
+                if (bx.z > 0) {
+                    __syncthreads();   // every wave is done with the previous box
+                    stage.commit(box, bx.w);
+                    __syncthreads();
+                }
+                {
+                    // the next angle's box is loaded while this one is gathered
+                    const int4 nx = uniform_box(b + 1);
+                    const bool st = b + 1 < a.nang && nx.z > 0;
+                    stage.issue(st ? proj + a.angle_bytes : a.stack, st ? nx.x : 0,
+                                st ? nx.y : 0, st ? nx.w : -1, a.row_bytes, a.Nv);
+                }
+                if (bx.z <= 0) {              // misses the detector, or pass 2
+                    need_hbm = need_hbm || bx.z < 0;
+                    continue;
+                }
+                const int bu0 = bx.x, bv0 = bx.y;
+
+                const R c = a.ang[2 * b], s = a.ang[2 * b + 1];
+                // the reference's S, T, U and u with nvcc's contractions
+                const R Sx = rfma(aX, c, aY * s);
+                const R T = rfma(aY, c, -(aX * s));
+                const R U = a.D / (a.D - Sx);
+                const R u = rfma(T, U, a.cX);
+                const bool act = inside && u >= (R)0 && u <= a.umax;
+                const R fu = rfract(u);
+                const int iu = act ? (int)u : 0;            // u >= 0: trunc = floor
+                const R W = act ? U * U * a.scales[b] : (R)0;
+                const R wu1 = W * fu, wu0 = W - wu1;
+                // LDS byte offset of sample (row iv, column iu) is
+                // iv*4*BOX_P + colb; inactive lanes sit at v = bv0, offset 0
+                const int colb = (act ? (iu - bu0) * 4 : 0) - bv0 * (4 * BOX_P);
+                const R Uv = act ? U : (R)0;
+                const R cz = act ? a.cZ : (R)bv0;
+                const R v_lo = rfma(az[0], Uv, cz), v_hi = rfma(az[ZR - 1], Uv, cz);
+                const R vmin = v_lo < v_hi ? v_lo : v_hi, vmax = v_lo < v_hi ? v_hi : v_lo;
+                const bool fast =
+                    __all(!act || (vmin >= (R)0.01 && vmax <= a.vmax - (R)0.01));
+                if (fast) {
+                    gather_run<true, ST::PAIRS, ZR, LB>(acc, az, box, Uv, cz, colb, wu0, wu1,
+                                                        a.vmax);
+                } else {
+                    // an opaque copy keeps the compiler from hoisting the
+                    // position arithmetic shared by both runs out of the
+                    // branch (which would hold all ZR positions live)
+                    R Uv2 = Uv, cz2 = cz;
+                    asm volatile("" : "+v"(Uv2), "+v"(cz2));
+                    gather_run<false, ST::PAIRS, ZR, LB>(acc, az, box, Uv2, cz2, colb, wu0, wu1,
+                                                         a.vmax);
+                }
+            }
+        }
+
+        // pass 2: angles gathered straight from HBM -- all of them for FP64,
+        // and those whose box exceeds the LDS tile for the staged types
+        if (need_hbm && __any(inside)) {
+            const char *proj = a.stack;
+            for (int b = 0; b < a.nang; ++b, proj += a.angle_bytes) {
+                if constexpr (STAGED) {
+                    const int4 bx = footprint(a.ang[2 * b], a.ang[2 * b + 1], cx0, cx1, cy0, cy1,
+                                              azlo, azhi, a.cX, a.cZ, a.D, a.Nu, a.Nv, BOX_H);
+                    if (bx.z >= 0) continue;
+                }
+                const R c = a.ang[2 * b], s = a.ang[2 * b + 1];
+                const R Sx = rfma(aX, c, aY * s);
+                const R T = rfma(aY, c, -(aX * s));
+                const R U = a.D / (a.D - Sx);
+                const R u = rfma(T, U, a.cX);
+                const bool act = inside && u >= (R)0 && u <= a.umax;
+                const R fu = rfract(u);
+                const int iu = act ? (int)u : 0;
+                const R W = act ? U * U * a.scales[b] : (R)0;
+                const R wu1 = W * fu, wu0 = W - wu1;
+                const R Uv = act ? U : (R)0;
+                const R cz = act ? a.cZ : (R)0;
+                const char *colp = proj + (size_t)iu * ST::BYTES;
 #pragma unroll
-                for (int k = 0; k < LB; ++k) acc[k0 + k] = rfma(W, val[k], acc[k0 + k]);
-#if TKV_BP_BATCH_BARRIER
-                // keep the next batch's loads behind this batch's arithmetic so
-                // at most LB gathers per lane are in flight (bounds the VGPRs)
-                __builtin_amdgcn_sched_barrier(0);
-#endif
+                for (int k = 0; k < ZR; ++k) {
+                    const R v = rfma(az[k], Uv, cz);
+                    const bool ok = act && v >= (R)0 && v <= a.vmax;
+                    const int iv = ok ? (int)v : 0;
+                    const R fv = rfract(v);
+                    const char *p0 = colp + (size_t)iv * a.row_bytes;
+                    // rows are zero-padded past Nu; row Nv-1 is only sampled
+                    // with fv = 0, and its missing neighbour is replaced by itself
+                    const int dn = iv + 1 < a.Nv ? (int)a.row_bytes : 0;
+                    const R q00 = ST::get(p0), q01 = ST::get(p0 + ST::BYTES);
+                    const R q10 = ST::get(p0 + dn), q11 = ST::get(p0 + dn + ST::BYTES);
+                    const R c0 = rfma(fv, q10 - q00, q00);
+                    const R c1 = rfma(fv, q11 - q01, q01);
+                    acc[k] = rfma(ok ? wu1 : (R)0, c1, rfma(ok ? wu0 : (R)0, c0, acc[k]));
+                }
             }
         }
     }
@@ -268,9 +568,9 @@ __global__ void __launch_bounds__(256) bp_kernel(BPArgs<typename Quad<Q>::R> a)
     R *out = a.vol + (size_t)y * a.Nx + x;
 #pragma unroll
     for (int k = 0; k < ZR; ++k) {
-        int z = z0 + k;
+        const int z = z0 + k;
         if (z < a.Nz) {
-            R r = inside ? acc[k] * a.out_scale : (R)0;
+            const R r = inside ? acc[k] * a.out_scale : (R)0;
             if (a.accumulate)
                 out[(size_t)z * plane] += r;
             else
@@ -279,44 +579,39 @@ __global__ void __launch_bounds__(256) bp_kernel(BPArgs<typename Quad<Q>::R> a)
     }
 }
 
-template <typename Q>
-constexpr int zrun_of()
+int bp_zrun(int dtype)
 {
-    return sizeof(typename Quad<Q>::R) == 8 ? 8 : TKV_BP_ZR;
-}
-
-int bp_zrun(int cell_dtype)
-{
-    switch (cell_dtype) {
-    case FDK_F64: return zrun_of<quad_f64>();
-    case FDK_F16: return zrun_of<quad_f16>();
-    case FDK_BF16: return zrun_of<quad_bf16>();
-    default: return zrun_of<quad_f32>();
-    }
-}
-
-int cell_bytes_of(int cell_dtype)
-{
-    switch (cell_dtype) {
-    case FDK_F32: return Quad<quad_f32>::BYTES;
-    case FDK_F16: return Quad<quad_f16>::BYTES;
-    case FDK_BF16: return Quad<quad_bf16>::BYTES;
-    case FDK_F64: return Quad<quad_f64>::BYTES;
+    switch (dtype) {
+    case FDK_F32: return Store<st_f32>::ZR;
+    case FDK_F16: return Store<st_f16>::ZR;
+    case FDK_BF16: return Store<st_bf16>::ZR;
+    case FDK_F64: return Store<st_f64>::ZR;
     default: return 0;
     }
 }
 
-template <typename Q>
+int sample_bytes_of(int dtype)
+{
+    switch (dtype) {
+    case FDK_F32: return 4;
+    case FDK_F16: return 2;
+    case FDK_BF16: return 2;
+    case FDK_F64: return 8;
+    default: return 0;
+    }
+}
+
+template <typename S>
 static int launch_bp_t(const BPLaunch &L, hipStream_t st)
 {
-    using R = typename Quad<Q>::R;
-    constexpr int ZR = zrun_of<Q>();
+    using R = typename Store<S>::R;
+    constexpr int ZR = Store<S>::ZR;
+    if (L.pitch < stack_pitch(L.Nu))
+        return fail(FDK_ERR_INVALID_ARGUMENT, "backproject: stack pitch below the padded width");
     BPArgs<R> a;
-    a.cells = (const char *)L.cells;
-    a.row_bytes = (uint32_t)((size_t)L.pitch * Quad<Q>::BYTES);
+    a.stack = (const char *)L.stack;
+    a.row_bytes = (uint32_t)((size_t)L.pitch * Store<S>::BYTES);
     a.angle_bytes = (size_t)a.row_bytes * L.Nv;
-    if (a.angle_bytes > 0xffffffffull || a.row_bytes >= (1u << 24))
-        return fail(FDK_ERR_UNSUPPORTED, "backproject: projection larger than 4 GiB");
     a.ang = (const R *)L.ang;
     a.scales = (const R *)L.scales;
     a.ax = (const R *)L.ax;
@@ -326,6 +621,8 @@ static int launch_bp_t(const BPLaunch &L, hipStream_t st)
     a.Nx = L.Nx;
     a.Ny = L.Ny;
     a.Nz = L.Nz;
+    a.Nu = L.Nu;
+    a.Nv = L.Nv;
     a.nang = L.nang;
     a.cX = (R)L.cX;
     a.cZ = (R)L.cZ;
@@ -337,25 +634,25 @@ static int launch_bp_t(const BPLaunch &L, hipStream_t st)
     a.accumulate = L.accumulate;
     a.tiles_x = (L.Nx + 15) / 16;
     a.tiles_xy = a.tiles_x * ((L.Ny + 15) / 16);
-    a.nzrun = (L.Nz + ZR - 1) / ZR;
-    long long nb = (long long)a.tiles_xy * a.nzrun;
+    const long long nzrun = (L.Nz + ZR - 1) / ZR;
+    const long long nb = (long long)a.tiles_xy * nzrun;
     if (nb == 0) return FDK_OK;
     if (nb > 0x7fffffffLL) return fail(FDK_ERR_UNSUPPORTED, "backproject: volume too large");
     a.nblocks = (int)nb;
-    hipLaunchKernelGGL((bp_kernel<Q, ZR, (ZR < TKV_BP_LB ? ZR : TKV_BP_LB)>), dim3((unsigned)nb),
+    hipLaunchKernelGGL((bp_kernel<S, ZR, (ZR < TKV_BP_LB ? ZR : TKV_BP_LB)>), dim3((unsigned)nb),
                        dim3(256), 0, st, a);
     TKV_HIP(hipGetLastError());
     return FDK_OK;
 }
 
-int launch_backproject(int cell_dtype, const BPLaunch &a, hipStream_t st)
+int launch_backproject(int dtype, const BPLaunch &a, hipStream_t st)
 {
-    switch (cell_dtype) {
-    case FDK_F32: return launch_bp_t<quad_f32>(a, st);
-    case FDK_F16: return launch_bp_t<quad_f16>(a, st);
-    case FDK_BF16: return launch_bp_t<quad_bf16>(a, st);
-    case FDK_F64: return launch_bp_t<quad_f64>(a, st);
-    default: return fail(FDK_ERR_INVALID_ARGUMENT, "backproject: unknown cell type");
+    switch (dtype) {
+    case FDK_F32: return launch_bp_t<st_f32>(a, st);
+    case FDK_F16: return launch_bp_t<st_f16>(a, st);
+    case FDK_BF16: return launch_bp_t<st_bf16>(a, st);
+    case FDK_F64: return launch_bp_t<st_f64>(a, st);
+    default: return fail(FDK_ERR_INVALID_ARGUMENT, "backproject: unknown storage type");
     }
 }
 

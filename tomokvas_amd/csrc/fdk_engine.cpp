@@ -8,8 +8,8 @@
 //    slab callback slab by slab (the reference launches once per Z-slab,
 //    kernel.cu:819-888), so slab size never limits the kernel's parallelism;
 //  * geometry tables and the filter spectrum are built in FP64 on the host;
-//  * the filtered stack is repacked into 2x2 bilinear cells, in FP32, FP16
-//    (per-projection scale) or BF16;
+//  * the filtered stack is repacked angle-major into the storage type, FP32,
+//    FP16 (per-projection scale) or BF16, with zero-padded rows;
 //  * destroy frees only this context's memory (no device reset).
 #include "fdk_internal.hpp"
 
@@ -52,7 +52,7 @@ struct fdk_ctx {
     // engine buffers (initialized)
     bool initialized = false;
     float *d_sino = nullptr, *d_filt = nullptr, *d_vol = nullptr, *d_partial = nullptr;
-    void *d_cells = nullptr;
+    void *d_stack = nullptr;
     float *d_scales = nullptr;
     unsigned *d_maxabs = nullptr;
     void *d_scratch = nullptr;
@@ -80,7 +80,7 @@ void free_tables(fdk_ctx *c)
 void free_engine(fdk_ctx *c)
 {
     dfree(c->d_sino); dfree(c->d_filt); dfree(c->d_vol); dfree(c->d_partial);
-    dfree(c->d_cells); dfree(c->d_scales); dfree(c->d_maxabs); dfree(c->d_scratch);
+    dfree(c->d_stack); dfree(c->d_scales); dfree(c->d_maxabs); dfree(c->d_scratch);
     c->scratch_bytes = 0;
     c->initialized = false;
 }
@@ -211,11 +211,7 @@ int need_prepared(fdk_ctx *c)
     return set_device(c);
 }
 
-int cell_bytes(int dtype) { return tkv::cell_bytes_of(dtype); }
-
-// cell rows are padded to a multiple of 64 cells (256-byte aligned rows for
-// FP16 quads); the padding cells are zero
-int cell_pitch(int Nu) { return ((Nu + 63) / 64) * 64; }
+int sample_bytes(int dtype) { return tkv::sample_bytes_of(dtype); }
 
 double out_scale(const fdk_ctx *c, int dtype)
 {
@@ -248,12 +244,12 @@ int stage_filter(fdk_ctx *c, int dtype, const void *in, void *out, int nang, int
     return tkv::launch_filter(f64 ? 8 : 4, L, st);
 }
 
-int stage_backproject(fdk_ctx *c, int cell_dtype, const void *cells, const void *scales,
+int stage_backproject(fdk_ctx *c, int store_dtype, const void *stack, const void *scales,
                       int ang0, int nang, void *vol, int accumulate, hipStream_t st)
 {
-    const bool f64 = cell_dtype == FDK_F64;
+    const bool f64 = store_dtype == FDK_F64;
     tkv::BPLaunch L;
-    L.cells = cells;
+    L.stack = stack;
     L.scales = scales ? scales : (f64 ? (const void *)c->d_one64 : (const void *)c->d_one32);
     L.ang = f64 ? (const void *)(c->d_ang64 + 2 * ang0) : (const void *)(c->d_ang32 + 2 * ang0);
     L.ax = f64 ? (const void *)c->d_ax64 : (const void *)c->d_ax32;
@@ -263,17 +259,17 @@ int stage_backproject(fdk_ctx *c, int cell_dtype, const void *cells, const void 
     L.Nu = c->g.Nu;
     L.Nv = c->g.Nv;
     L.nang = nang;
-    L.pitch = cell_pitch(c->g.Nu);
+    L.pitch = tkv::stack_pitch(c->g.Nu);
     L.Nx = c->g.Nx;
     L.Ny = c->g.Ny;
     L.Nz = c->g.Nz;
     L.cX = c->g.cX;
     L.cZ = c->g.cZ;
     L.D = c->g.D;
-    L.out_scale = out_scale(c, cell_dtype);
+    L.out_scale = out_scale(c, store_dtype);
     L.maskR2 = 0.25 * (double)c->g.Nu * (double)c->g.Nu;
     L.accumulate = accumulate;
-    return tkv::launch_backproject(cell_dtype, L, st);
+    return tkv::launch_backproject(store_dtype, L, st);
 }
 
 int check_angles(fdk_ctx *c, int ang0, int nang)
@@ -461,9 +457,9 @@ int fdk_set_storage(fdk_ctx *c, int dtype)
     if (dtype != FDK_F32 && dtype != FDK_F16 && dtype != FDK_BF16)
         return fail(FDK_ERR_INVALID_ARGUMENT, "storage must be F32, F16 or BF16");
     if (c->initialized && dtype != c->storage) {
-        dfree(c->d_cells);
-        size_t bytes = fdk_cells_bytes(c, dtype, c->g.Nang);
-        int rc = dalloc(&c->d_cells, bytes, "cell stack");
+        dfree(c->d_stack);
+        size_t bytes = fdk_stack_bytes(c, dtype, c->g.Nang);
+        int rc = dalloc(&c->d_stack, bytes, "stored stack");
         if (rc) return rc;
     }
     c->storage = dtype;
@@ -476,11 +472,14 @@ int fdk_prepare(fdk_ctx *c)
     return prepare(c);
 }
 
-size_t fdk_cells_bytes(fdk_ctx *c, int cell_dtype, int nang)
+size_t fdk_stack_bytes(fdk_ctx *c, int store_dtype, int nang)
 {
     if (!c) return 0;
-    return (size_t)cell_bytes(cell_dtype) * (size_t)cell_pitch(c->g.Nu) * (size_t)c->g.Nv *
-           (size_t)nang;
+    // nang projections plus tkv::STACK_TAIL_ROWS spare rows: the backprojector's
+    // box loads past the last detector row are range-checked to zero, and the
+    // spare rows keep even those addresses inside the allocation
+    const size_t row = (size_t)sample_bytes(store_dtype) * (size_t)tkv::stack_pitch(c->g.Nu);
+    return row * ((size_t)c->g.Nv * (size_t)nang + tkv::STACK_TAIL_ROWS);
 }
 
 int fdk_initialize(fdk_ctx *c)
@@ -495,7 +494,7 @@ int fdk_initialize(fdk_ctx *c)
     const size_t slab = (size_t)g.Nx * g.Ny * g.B;
     if ((rc = dalloc((void **)&c->d_sino, nsino * sizeof(float), "projection stack")) ||
         (rc = dalloc((void **)&c->d_filt, nsino * sizeof(float), "filtered stack")) ||
-        (rc = dalloc(&c->d_cells, fdk_cells_bytes(c, c->storage, g.Nang), "cell stack")) ||
+        (rc = dalloc(&c->d_stack, fdk_stack_bytes(c, c->storage, g.Nang), "stored stack")) ||
         (rc = dalloc((void **)&c->d_vol, nvol * sizeof(float), "volume")) ||
         (rc = dalloc((void **)&c->d_scales, g.Nang * sizeof(float), "scales")) ||
         (rc = dalloc((void **)&c->d_maxabs, g.Nang * sizeof(unsigned), "maxabs"))) {
@@ -544,11 +543,11 @@ int fdk_calculate_reconstruction(fdk_ctx *c, fdk_get_block_fn get_block, void *u
     if ((rc = stage_filter(c, FDK_F32, c->d_sino, c->d_filt, g.Nang, 1, 0, c->d_maxabs, st)))
         return rc;
     TKV_HIP(hipEventRecord(c->ev[1], st));
-    if ((rc = tkv::launch_pack(FDK_F32, c->d_filt, g.Nu, g.Nv, g.Nang, cell_pitch(g.Nu),
-                               c->d_maxabs, c->storage, c->d_cells, c->d_scales, st)))
+    if ((rc = tkv::launch_pack(FDK_F32, c->d_filt, g.Nu, g.Nv, g.Nang, tkv::stack_pitch(g.Nu),
+                               c->d_maxabs, c->storage, c->d_stack, c->d_scales, st)))
         return rc;
     TKV_HIP(hipEventRecord(c->ev[2], st));
-    if ((rc = stage_backproject(c, c->storage, c->d_cells, c->d_scales, 0, g.Nang, c->d_vol, 0,
+    if ((rc = stage_backproject(c, c->storage, c->d_stack, c->d_scales, 0, g.Nang, c->d_vol, 0,
                                 st)))
         return rc;
     TKV_HIP(hipEventRecord(c->ev[3], st));
@@ -624,25 +623,25 @@ int fdk_filter(fdk_ctx *c, int dtype, void *d_sino, int nang, int weight, int ad
 }
 
 int fdk_pack(fdk_ctx *c, int dtype_in, const void *d_filtered, int nang,
-             const uint32_t *d_maxabs, int cell_dtype, void *d_cells, void *d_scales,
+             const uint32_t *d_maxabs, int store_dtype, void *d_stack, void *d_scales,
              void *stream)
 {
     int rc = need_prepared(c);
     if (rc) return rc;
-    if (!d_filtered || !d_cells || !d_scales)
+    if (!d_filtered || !d_stack || !d_scales)
         return fail(FDK_ERR_INVALID_ARGUMENT, "pack: null buffer");
-    return tkv::launch_pack(dtype_in, d_filtered, c->g.Nu, c->g.Nv, nang, cell_pitch(c->g.Nu),
-                            d_maxabs, cell_dtype, d_cells, d_scales, pick(c, stream));
+    return tkv::launch_pack(dtype_in, d_filtered, c->g.Nu, c->g.Nv, nang, tkv::stack_pitch(c->g.Nu),
+                            d_maxabs, store_dtype, d_stack, d_scales, pick(c, stream));
 }
 
-int fdk_backproject(fdk_ctx *c, int cell_dtype, const void *d_cells, const void *d_scales,
+int fdk_backproject(fdk_ctx *c, int store_dtype, const void *d_stack, const void *d_scales,
                     int ang0, int nang, void *d_vol, int accumulate, void *stream)
 {
     int rc = need_prepared(c);
     if (rc) return rc;
     if ((rc = check_angles(c, ang0, nang))) return rc;
-    if (!d_cells || !d_vol) return fail(FDK_ERR_INVALID_ARGUMENT, "backproject: null buffer");
-    return stage_backproject(c, cell_dtype, d_cells, d_scales, ang0, nang, d_vol, accumulate,
+    if (!d_stack || !d_vol) return fail(FDK_ERR_INVALID_ARGUMENT, "backproject: null buffer");
+    return stage_backproject(c, store_dtype, d_stack, d_scales, ang0, nang, d_vol, accumulate,
                              pick(c, stream));
 }
 

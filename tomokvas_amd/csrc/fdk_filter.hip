@@ -1,5 +1,5 @@
 // fdk_filter.hip -- cosine pre-weighting, LDS-resident Stockham FFT ramp filter,
-// and the repack of filtered rows into the backprojector's cell stack (gfx950).
+// and the repack of filtered rows into the backprojector's stored stack (gfx950).
 //
 // Reference path: RB_comma3DKernel (FBPG/kernel.cu:179-201), fConvolution with
 // cuFFT R2C/C2R per detector row (FBPG/kernel.cu:557-781), FiltrationKernel
@@ -675,130 +675,120 @@ int launch_tap_gradient(int real_bytes, const TapGradLaunch &a, hipStream_t st)
     return real_bytes == 8 ? launch_tapgrad_t<double>(a, st) : launch_tapgrad_t<float>(a, st);
 }
 
-// ---------------------------------------------------------------- repack to cells
-// filtered [v][beta][u] (R)  ->  quad cells [beta][v][pitch] of the bilinear
-// coefficients of the backprojector (Q = 0 outside [0,Nv) x [0,Nu)):
-//   A = Q(v,u), B = Q(v+1,u) - Q(v,u), C = Q(v,u+1) - Q(v,u),
-//   D = (Q(v+1,u+1) - Q(v+1,u)) - (Q(v,u+1) - Q(v,u)),   all divided by scale_beta.
-// Each thread walks PACK_VB consecutive rows of one column.
-template <typename Cell>
-struct QuadMake;
+// ---------------------------------------------------------------- repack
+// filtered [v][beta][u] (R)  ->  stored stack [beta][v][pitch] in the storage
+// type, samples divided by scale_beta, zero for u >= Nu.  A workgroup writes
+// four stack rows; a lane converts four consecutive samples per step.
+template <typename Out>
+struct Pack4;
 template <>
-struct QuadMake<float4> {
-    static __device__ __forceinline__ float4 make(float a, float b, float c, float d)
+struct Pack4<float> {
+    using V = float4;
+    static __device__ __forceinline__ V make(float a, float b, float c, float d)
     {
         return {a, b, c, d};
     }
 };
 template <>
-struct QuadMake<double4> {
-    static __device__ __forceinline__ double4 make(double a, double b, double c, double d)
+struct Pack4<double> {
+    using V = double4;
+    static __device__ __forceinline__ V make(double a, double b, double c, double d)
     {
         return {a, b, c, d};
     }
 };
-struct quad_h {
-    uint2 v;
-};
-struct quad_b {
-    uint2 v;
-};
+struct f16_t;
+struct bf16_t;
 template <>
-struct QuadMake<quad_h> {
+struct Pack4<f16_t> {
+    using V = uint2;
     static __device__ __forceinline__ unsigned h(float x)
     {
         return __half_as_ushort(__float2half_rn(x));
     }
-    static __device__ __forceinline__ quad_h make(float a, float b, float c, float d)
+    static __device__ __forceinline__ V make(float a, float b, float c, float d)
     {
-        return {{h(a) | (h(b) << 16), h(c) | (h(d) << 16)}};
+        return {h(a) | (h(b) << 16), h(c) | (h(d) << 16)};
     }
 };
 template <>
-struct QuadMake<quad_b> {
+struct Pack4<bf16_t> {
+    using V = uint2;
     static __device__ __forceinline__ unsigned h(float x)
     {
         return __bfloat16_as_ushort(__float2bfloat16(x));
     }
-    static __device__ __forceinline__ quad_b make(float a, float b, float c, float d)
+    static __device__ __forceinline__ V make(float a, float b, float c, float d)
     {
-        return {{h(a) | (h(b) << 16), h(c) | (h(d) << 16)}};
+        return {h(a) | (h(b) << 16), h(c) | (h(d) << 16)};
     }
 };
 
-constexpr int PACK_VB = 16;
-// FP16: the largest filtered magnitude of a projection maps to 2^13, so the
-// second difference D (at most 4x that) stays below the FP16 maximum 65504,
-// and values 2^27 below the maximum are still normal numbers.
+// FP16: the largest filtered magnitude of a projection maps to 2^13; values
+// 2^27 below it are still normal FP16 numbers.
 constexpr float F16_TARGET = 8192.0f;
 
-template <typename R, typename Cell, bool SCALED>
+template <typename R, typename Out, bool SCALED>
 __global__ void __launch_bounds__(256)
     pack_kernel(const R *__restrict__ filt, int Nu, int Nv, int nang, int pitch,
-                const unsigned *__restrict__ maxabs, Cell *__restrict__ cells,
+                const unsigned *__restrict__ maxabs, typename Pack4<Out>::V *__restrict__ stack,
                 R *__restrict__ scales)
 {
-    const int u = blockIdx.x * 256 + threadIdx.x;
-    const int beta = blockIdx.z;
-    const int v0 = blockIdx.y * PACK_VB;
-    if (u >= pitch) return;
+    const long long row = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= (long long)nang * Nv) return;
+    const int beta = (int)(row / Nv), v = (int)(row - (long long)beta * Nv);
     R inv = 1, sc = 1;
     if (SCALED) {
-        float m = __uint_as_float(maxabs[beta]);
+        const float m = __uint_as_float(maxabs[beta]);
         sc = m > 0 ? (R)(m / F16_TARGET) : (R)1;
         inv = (R)1 / sc;
     }
-    if (u == 0 && blockIdx.y == 0) scales[beta] = sc;
-    const size_t rstride = (size_t)nang * Nu;
-    const R *src = filt + (size_t)beta * Nu;
-    const bool c0 = u < Nu, c1 = u + 1 < Nu;
-    Cell *dst = cells + ((size_t)beta * Nv) * pitch + u;
-    R q00 = (c0 && v0 < Nv) ? src[(size_t)v0 * rstride + u] : (R)0;
-    R q01 = (c1 && v0 < Nv) ? src[(size_t)v0 * rstride + u + 1] : (R)0;
-    const int vend = min(v0 + PACK_VB, Nv);
-    for (int v = v0; v < vend; ++v) {
-        const bool rn = v + 1 < Nv;
-        R q10 = (c0 && rn) ? src[(size_t)(v + 1) * rstride + u] : (R)0;
-        R q11 = (c1 && rn) ? src[(size_t)(v + 1) * rstride + u + 1] : (R)0;
-        R dB = q10 - q00, dC = q01 - q00;
-        R dD = (q11 - q10) - dC;
-        dst[(size_t)v * pitch] = QuadMake<Cell>::make(q00 * inv, dB * inv, dC * inv, dD * inv);
-        q00 = q10;
-        q01 = q11;
+    const int lane = threadIdx.x & 63;
+    if (v == 0 && lane == 0) scales[beta] = sc;
+    const R *src = filt + ((size_t)v * nang + beta) * Nu;
+    typename Pack4<Out>::V *dst = stack + (size_t)row * (pitch / 4);
+    for (int g = lane; g < pitch / 4; g += 64) {
+        const int u = 4 * g;
+        R q[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) q[i] = u + i < Nu ? src[u + i] * inv : (R)0;
+        dst[g] = Pack4<Out>::make(q[0], q[1], q[2], q[3]);
     }
 }
 
 int launch_pack(int dtype_in, const void *filt, int Nu, int Nv, int nang, int pitch,
-                const unsigned *maxabs, int cell_dtype, void *cells, void *scales,
+                const unsigned *maxabs, int store_dtype, void *stack, void *scales,
                 hipStream_t st)
 {
     if (nang == 0 || Nv == 0) return FDK_OK;
-    if (nang > 65535) return fail(FDK_ERR_UNSUPPORTED, "pack: more than 65535 projections");
-    dim3 grid((pitch + 255) / 256, (Nv + PACK_VB - 1) / PACK_VB, nang);
+    if (pitch % 4 != 0) return fail(FDK_ERR_INVALID_ARGUMENT, "pack: pitch not a multiple of 4");
+    const long long rows = (long long)nang * Nv;
+    if ((rows + 3) / 4 > 0x7fffffffLL) return fail(FDK_ERR_UNSUPPORTED, "pack: stack too large");
+    dim3 grid((unsigned)((rows + 3) / 4));
     if (dtype_in == FDK_F64) {
-        if (cell_dtype != FDK_F64)
-            return fail(FDK_ERR_INVALID_ARGUMENT, "pack: F64 input needs F64 cells");
-        hipLaunchKernelGGL((pack_kernel<double, double4, false>), grid, dim3(256), 0, st,
-                           (const double *)filt, Nu, Nv, nang, pitch, maxabs, (double4 *)cells,
+        if (store_dtype != FDK_F64)
+            return fail(FDK_ERR_INVALID_ARGUMENT, "pack: F64 input needs F64 storage");
+        hipLaunchKernelGGL((pack_kernel<double, double, false>), grid, dim3(256), 0, st,
+                           (const double *)filt, Nu, Nv, nang, pitch, maxabs, (double4 *)stack,
                            (double *)scales);
     } else {
         const float *f = (const float *)filt;
-        switch (cell_dtype) {
+        switch (store_dtype) {
         case FDK_F32:
-            hipLaunchKernelGGL((pack_kernel<float, float4, false>), grid, dim3(256), 0, st, f,
-                               Nu, Nv, nang, pitch, maxabs, (float4 *)cells, (float *)scales);
+            hipLaunchKernelGGL((pack_kernel<float, float, false>), grid, dim3(256), 0, st, f, Nu,
+                               Nv, nang, pitch, maxabs, (float4 *)stack, (float *)scales);
             break;
         case FDK_F16:
             if (!maxabs) return fail(FDK_ERR_INVALID_ARGUMENT, "pack: FP16 needs maxabs");
-            hipLaunchKernelGGL((pack_kernel<float, quad_h, true>), grid, dim3(256), 0, st, f,
-                               Nu, Nv, nang, pitch, maxabs, (quad_h *)cells, (float *)scales);
+            hipLaunchKernelGGL((pack_kernel<float, f16_t, true>), grid, dim3(256), 0, st, f, Nu,
+                               Nv, nang, pitch, maxabs, (uint2 *)stack, (float *)scales);
             break;
         case FDK_BF16:
-            hipLaunchKernelGGL((pack_kernel<float, quad_b, false>), grid, dim3(256), 0, st, f,
-                               Nu, Nv, nang, pitch, maxabs, (quad_b *)cells, (float *)scales);
+            hipLaunchKernelGGL((pack_kernel<float, bf16_t, false>), grid, dim3(256), 0, st, f,
+                               Nu, Nv, nang, pitch, maxabs, (uint2 *)stack, (float *)scales);
             break;
         default:
-            return fail(FDK_ERR_INVALID_ARGUMENT, "pack: F32 input needs F32/F16/BF16 cells");
+            return fail(FDK_ERR_INVALID_ARGUMENT, "pack: F32 input needs F32/F16/BF16 storage");
         }
     }
     TKV_HIP(hipGetLastError());

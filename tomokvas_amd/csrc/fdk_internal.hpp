@@ -4,10 +4,10 @@
 // configuration without tiling):
 //   input stack   float [v][beta][u]          as the host hands it over
 //                 (sinogram-major, FBPG/mucat_file.cpp:244-267)
-//   cell stack    Quad  [beta][v][pitch]      filtered projections, angle-major;
-//                 quad(beta,v,u) = the four bilinear coefficients of the
-//                 2x2 neighbourhood at (v,u) (fdk_filter.hip, pack_kernel),
-//                 so one aligned load serves one voxel-update.
+//   stored stack  S     [beta][v][pitch]      filtered projections, angle-major,
+//                 in the storage type S (FP32, FP16 / scale_beta, BF16, FP64);
+//                 pitch = stack_pitch(Nu) >= Nu + 68, zero past column Nu-1,
+//                 so the backprojector's staged boxes stay inside a row.
 //   volume        float [z][y][x]             the full volume, slabs are z-ranges
 #pragma once
 
@@ -79,14 +79,14 @@ int launch_weight(int real_bytes, void *sino, int Nu, int Nv, int nang, float cX
                   float D, hipStream_t st);
 int launch_filter(int real_bytes, const FilterLaunch &a, hipStream_t st);
 int launch_pack(int dtype_in, const void *filt, int Nu, int Nv, int nang, int pitch,
-                const unsigned *maxabs, int cell_dtype, void *cells, void *scales,
+                const unsigned *maxabs, int store_dtype, void *stack, void *scales,
                 hipStream_t st);
 int launch_accumulate(float *dst, const float *src, size_t n, hipStream_t st);
 int launch_zero_u32(unsigned *dst, size_t n, hipStream_t st);
 int launch_compare(const float *a, const float *b, size_t n, double *d_out3, hipStream_t st);
 
 struct BPLaunch {
-    const void *cells;      // [nang][Nv][pitch]
+    const void *stack;      // [nang][Nv][pitch] in the storage type
     const void *scales;     // per local angle dequantisation factor (Real)
     const void *ang;        // per local angle (cos, sin) pairs (Real)
     const void *ax, *ay, *az;  // coordinate tables (Real), az padded to the z-run
@@ -96,9 +96,11 @@ struct BPLaunch {
     double cX, cZ, D, out_scale, maskR2;
     int accumulate;
 };
-int launch_backproject(int cell_dtype, const BPLaunch &a, hipStream_t st);
-int bp_zrun(int cell_dtype);
-int cell_bytes_of(int cell_dtype);   // bytes of one quad cell
+int launch_backproject(int store_dtype, const BPLaunch &a, hipStream_t st);
+int bp_zrun(int store_dtype);
+int sample_bytes_of(int store_dtype);   // bytes of one stored sample
+int stack_pitch(int Nu);                // row pitch of the stored stack, samples
+constexpr int STACK_TAIL_ROWS = 96;     // spare rows after the last projection (>= box rows)
 
 struct FPLaunch {           // adjoint of the backprojector
     const void *vol;        // [Nz][Ny][Nx]

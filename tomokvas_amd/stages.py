@@ -73,35 +73,35 @@ class Stages:
                                      int(weight), int(adjoint), mp, _stream()), "fdk_filter")
         return sino
 
-    def cells_bytes(self, cell_dtype, nang=None):
-        return self.L.fdk_cells_bytes(self.fbp.handle, cell_dtype,
+    def stack_bytes(self, store_dtype, nang=None):
+        return self.L.fdk_stack_bytes(self.fbp.handle, store_dtype,
                                       self.Nang if nang is None else nang)
 
-    def pack(self, filtered, cell_dtype, maxabs=None):
-        """Returns (cells uint8 tensor, scales tensor)."""
+    def pack(self, filtered, store_dtype, maxabs=None):
+        """Returns (stored stack as a uint8 tensor, scales tensor)."""
         torch = _torch()
         _check_dev(filtered)
         nang = filtered.shape[1]
-        nbytes = self.cells_bytes(cell_dtype, nang)
-        cells = torch.empty(nbytes, dtype=torch.uint8, device=filtered.device)
-        sdt = torch.float64 if cell_dtype == F64 else torch.float32
+        nbytes = self.stack_bytes(store_dtype, nang)
+        stack = torch.empty(nbytes, dtype=torch.uint8, device=filtered.device)
+        sdt = torch.float64 if store_dtype == F64 else torch.float32
         scales = torch.empty(nang, dtype=sdt, device=filtered.device)
         mp = maxabs.data_ptr() if maxabs is not None else None
         _lib.check(self.L.fdk_pack(self.fbp.handle, _dtype_code(filtered), filtered.data_ptr(),
-                                   nang, mp, cell_dtype, cells.data_ptr(), scales.data_ptr(),
+                                   nang, mp, store_dtype, stack.data_ptr(), scales.data_ptr(),
                                    _stream()), "fdk_pack")
-        return cells, scales
+        return stack, scales
 
-    def backproject(self, cells, cell_dtype, scales=None, ang0=0, nang=None, vol=None,
+    def backproject(self, stack, store_dtype, scales=None, ang0=0, nang=None, vol=None,
                     accumulate=False):
         torch = _torch()
         nang = self.Nang - ang0 if nang is None else nang
-        vdt = torch.float64 if cell_dtype == F64 else torch.float32
+        vdt = torch.float64 if store_dtype == F64 else torch.float32
         if vol is None:
-            vol = torch.zeros(self.vol_shape, dtype=vdt, device=cells.device)
+            vol = torch.zeros(self.vol_shape, dtype=vdt, device=stack.device)
         _check_dev(vol, self.vol_shape)
         sp = scales.data_ptr() if scales is not None else None
-        _lib.check(self.L.fdk_backproject(self.fbp.handle, cell_dtype, cells.data_ptr(), sp, ang0,
+        _lib.check(self.L.fdk_backproject(self.fbp.handle, store_dtype, stack.data_ptr(), sp, ang0,
                                           nang, vol.data_ptr(), int(accumulate), _stream()),
                    "fdk_backproject")
         return vol
