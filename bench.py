@@ -32,6 +32,9 @@ METRIC = ("backprojection GUPS + FDK seconds @1024³: FP16-store vs FP32, "
           "rel-RMSE≤1e-3")
 L2_PEAK_GBS = 8 * 16 * 128 * 2.4  # 8 XCD x 16 channels x 128 B/clk x 2.4 GHz = 39321.6 GB/s
 HBM_PEAK_GBS = 8000.0
+# non-packed VALU issue: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz (one wave64 instruction
+# per SIMD every 4 clocks)
+VALU_LANE_OPS = 256 * 4 * 16 * 2.4e9
 NAMES = {_lib.F32: "f32", _lib.F16: "f16", _lib.BF16: "bf16"}
 CELL_BYTES = {_lib.F32: 4, _lib.F16: 2, _lib.BF16: 2}   # bytes per stored sample
 
@@ -50,17 +53,18 @@ def hip():
     return h
 
 
-def read_traffic(workload: str, storage: str):
-    """HBM bytes per backprojection launch measured by rocprofv3 --pmc
-    (profiles/bp_traffic_<workload>.txt, lines 'storage fetch_bytes write_bytes')."""
+def read_profile(workload: str, storage: str):
+    """(HBM bytes, VALU instructions per voxel-update) of one backprojection launch,
+    measured by rocprofv3 --pmc (profiles/bp_traffic_<workload>.txt, lines
+    'storage fetch_bytes write_bytes valu_per_update'); (None, None) if absent."""
     path = os.path.join(ROOT, "profiles", f"bp_traffic_{workload}.txt")
     if not os.path.exists(path):
-        return None
+        return None, None
     for line in open(path):
         parts = line.split()
-        if len(parts) == 3 and parts[0] == storage and not line.startswith("#"):
-            return float(parts[1]) + float(parts[2])
-    return None
+        if len(parts) == 4 and parts[0] == storage and not line.startswith("#"):
+            return float(parts[1]) + float(parts[2]), float(parts[3])
+    return None, None
 
 
 def sub_cube(g, seed: int, n: int = 64):
@@ -146,6 +150,10 @@ def main():
     ap.add_argument("--storages", default="f32,f16,bf16")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--calibrate", action="store_true",
+                    help="after the timed steps, run accumulate_kernel once over 2^28 floats "
+                         "(reads 2 GiB, writes 1 GiB): the reference launch for scaling "
+                         "rocprofv3 FETCH_SIZE/WRITE_SIZE readings to bytes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the FDK path "
@@ -228,14 +236,22 @@ def main():
     s_bytes = CELL_BYTES[{"f32": _lib.F32, "f16": _lib.F16, "bf16": _lib.BF16}[head]]
     gather = 4 * s_bytes * upd                      # 2x2 taps per voxel-update
     achieved = gather / (r["bp_ms"] * 1e-3) / 1e9   # GB/s
-    traffic = read_traffic(args.workload, head)
+    traffic, valu_per_update = read_profile(args.workload, head)
+    t_bp = r["bp_ms"] * 1e-3
     roofline = {
         "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-        "kernel": "bp_kernel", "algorithmic_bytes": gather,
+        "kernel": f"bp_kernel<{head}>", "algorithmic_bytes": gather,
         "l2_peak": L2_PEAK_GBS, "frac_l2": achieved / L2_PEAK_GBS,
         "compulsory_hbm_bytes": g.Nang * g.Nv * g.Nu * s_bytes + 4 * g.Nx * g.Ny * g.Nz,
+        "hbm_traffic_rate_gbs": traffic / t_bp / 1e9 if traffic else None,
     }
+    if valu_per_update:
+        # the gathers are served from LDS: the kernel is bound by VALU issue
+        t_valu = upd * valu_per_update / VALU_LANE_OPS
+        roofline["valu"] = {"instr_per_update": valu_per_update,
+                            "peak_lane_instr_per_s": VALU_LANE_OPS,
+                            "t_min_ms": 1e3 * t_valu, "frac": t_valu / t_bp}
     out = {
         "metric": METRIC, "value": r["gups"], "unit": "GUPS", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": r["fdk_ms"],
@@ -260,6 +276,15 @@ def main():
         out["cpu_baseline"] = cpu
         out["rel_rmse"].update(errs)
         log(f"cpu leg {time.perf_counter() - t0:.1f} s: {cpu['value']:.3f} GUPS, errors {errs}")
+    if args.calibrate:
+        n = 1 << 28
+        dst = fbp.device_sinogram_ptr()
+        assert g.Nu * g.Nv * g.Nang >= 3 * n, "calibration needs 3 GiB of projection buffer"
+        _lib.check(L.fdk_accumulate(C.c_void_p(dst), C.c_void_p(dst + 4 * n), n, None),
+                   "calibration")
+        hipm.hipDeviceSynchronize()
+        out["calibration"] = {"kernel": "accumulate_kernel", "floats": n,
+                              "read_bytes": 8 * n, "write_bytes": 4 * n}
     fbp.close()
     print(json.dumps(out), flush=True)
 

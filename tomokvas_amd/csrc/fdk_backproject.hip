@@ -61,6 +61,11 @@
 #ifndef TKV_BP_F16_PAIRS
 #define TKV_BP_F16_PAIRS 1
 #endif
+//   TKV_BP_DBUF                    two LDS boxes used alternately (one barrier
+//                                  per angle instead of two)
+#ifndef TKV_BP_DBUF
+#define TKV_BP_DBUF 0
+#endif
 
 namespace tkv {
 
@@ -383,7 +388,8 @@ __global__ void __launch_bounds__(256) bp_kernel(BPArgs<typename Store<S>::R> a)
     using ST = Store<S>;
     constexpr bool STAGED = ST::STAGED;
     constexpr int BOX_H = box_rows<ZR>();
-    __shared__ float box[STAGED ? BOX_P * BOX_H : 1];
+    constexpr int NBUF = TKV_BP_DBUF ? 2 : 1;
+    __shared__ float boxmem[STAGED ? NBUF * BOX_P * BOX_H : 1];
     __shared__ int4 boxes[STAGED ? 128 : 1];   // footprints of two groups of 64 angles
 
     // XCD-aware block order: hardware dispatches workgroup i to XCD i % 8; the
@@ -459,6 +465,7 @@ __global__ void __launch_bounds__(256) bp_kernel(BPArgs<typename Store<S>::R> a)
                             b0.z > 0 ? b0.w : -1, a.row_bytes, a.Nv);
             }
             const char *proj = a.stack;
+            int nstaged = 0;
             for (int b = 0; b < a.nang; ++b, proj += a.angle_bytes) {
                 if ((b & 63) == 0 && b > 0) {
                     // the slot of group b/64 - 1 is free once every wave is here
@@ -467,10 +474,15 @@ __global__ void __launch_bounds__(256) bp_kernel(BPArgs<typename Store<S>::R> a)
                     __syncthreads();
                 }
                 const int4 bx = uniform_box(b);
+                // with two boxes, staged angle s uses box s % 2: it was last
+                // read by staged angle s-2, which every wave finished before
+                // the barrier of staged angle s-1
+                float *box = boxmem + (NBUF == 2 ? (nstaged & 1) * BOX_P * BOX_H : 0);
                 if (bx.z > 0) {
-                    __syncthreads();   // every wave is done with the previous box
+                    if (NBUF == 1) __syncthreads();   // every wave is done with the box
                     stage.commit(box, bx.w);
                     __syncthreads();
+                    ++nstaged;
                 }
                 {
                     // the next angle's box is loaded while this one is gathered
