@@ -25,7 +25,7 @@ def fma(a, b, c):  # float32 fma via float64 (exact product)
 
 BOX_W, ALIGN = 64, 4
 # rows of the LDS box per z-run length (box_rows in the kernel)
-BOX_ROWS = {16: 64, 24: 80, 32: 96}
+BOX_ROWS = {16: 64, 24: 80, 32: 96, 48: 128}
 
 
 def footprint(c, s, x0, x1, y0, y1, azlo, azhi, cX, cZ, D, Nu, Nv, box_h):
@@ -49,7 +49,8 @@ def footprint(c, s, x0, x1, y0, y1, azlo, azhi, cX, cZ, D, Nu, Nv, box_h):
     bw, bh = bu1 - bu0 + 1, bv1 - bv0 + 1
     if bw <= 0 or bh <= 0: return (0, 0, 0, 0)
     if bw >= BOX_W or bh >= box_h: return (0, 0, -1, 0)
-    return (bu0, bv0, bw, bh)
+    inner = bool(vlo >= f32(0.02) and vhi <= f32(Nv - 1) - f32(0.02))
+    return (bu0, bv0, bw | (0x10000 if inner else 0), bh)
 
 def check(g, ZR=32, tile_stride=1, ang_stride=1):
     box_h = BOX_ROWS[ZR]
@@ -68,7 +69,7 @@ def check(g, ZR=32, tile_stride=1, ang_stride=1):
     maskR2 = 0.25 * Nu * Nu
     tiles_x, tiles_y = (g.Nx + 15) // 16, (g.Ny + 15) // 16
     nzr = (g.Nz + ZR - 1) // ZR
-    stats = dict(staged=0, empty=0, hbm=0, taps=0, maxbw=0, maxbh=0)
+    stats = dict(staged=0, empty=0, hbm=0, inner=0, taps=0, maxbw=0, maxbh=0)
     for zr in range(nzr):
         azr = az[zr * ZR: zr * ZR + ZR]
         azlo, azhi = min(azr[0], azr[-1]), max(azr[0], azr[-1])
@@ -81,7 +82,8 @@ def check(g, ZR=32, tile_stride=1, ang_stride=1):
                 x0, x1 = ax[tx * 16], ax[min(tx * 16 + 15, g.Nx - 1)]
                 y0, y1 = ay[ty * 16], ay[min(ty * 16 + 15, g.Ny - 1)]
                 for b in range(0, g.Nang, ang_stride):
-                    bu0, bv0, bw, bh = footprint(cs[b], ss[b], x0, x1, y0, y1, azlo, azhi, cX, cZ, D, Nu, Nv, box_h)
+                    bu0, bv0, bwf, bh = footprint(cs[b], ss[b], x0, x1, y0, y1, azlo, azhi, cX, cZ, D, Nu, Nv, box_h)
+                    bw, inner = (bwf & 0xffff, bool(bwf & 0x10000)) if bwf > 0 else (bwf, False)
                     if bw == 0:
                         # nothing may land on the detector
                         kind = 'empty'
@@ -107,6 +109,11 @@ def check(g, ZR=32, tile_stride=1, ang_stride=1):
                     iv = np.floor(v).astype(np.int64)
                     col = iu - bu0
                     assert col.min() >= 0 and col.max() + 1 <= bw and bw <= 62, ("col", zr, tx, ty, b, col.min(), col.max(), bw)
+                    if inner:
+                        # the kernel then skips the per-slice range test
+                        assert ok.all() and v.min() >= 0.01 and v.max() <= Nv - 1.01, \
+                            ("inner", zr, tx, ty, b, float(v.min()), float(v.max()))
+                        stats['inner'] += 1
                     rows = (iv - bv0)[ok]
                     if rows.size:
                         assert rows.min() >= 0 and rows.max() + 1 <= bh, ("row", zr, tx, ty, b, rows.min(), rows.max(), bh)

@@ -75,7 +75,7 @@ namespace tkv {
 constexpr int BOX_W = 64;
 constexpr int BOX_P = TKV_BP_BOXP;
 template <int ZR>
-constexpr int box_rows() { return ZR > 24 ? 96 : ZR > 16 ? 80 : 64; }
+constexpr int box_rows() { return ZR > 32 ? 128 : ZR > 24 ? 96 : ZR > 16 ? 80 : 64; }
 static_assert(box_rows<64>() <= STACK_TAIL_ROWS, "stored stack tail shorter than a box");
 constexpr int BOX_ALIGN = 4;   // box columns start on a multiple of 4 samples
 static_assert(BOX_P >= BOX_W && BOX_P % 4 == 0, "box pitch");
@@ -161,6 +161,8 @@ struct Store<st_f64> {
     static __device__ __forceinline__ float half(unsigned, int) { return 0.f; }
 };
 
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+
 template <typename R>
 __device__ __forceinline__ R rfma(R a, R b, R c);
 template <>
@@ -176,7 +178,8 @@ template <>
 __device__ __forceinline__ double rfract(double x) { return __builtin_amdgcn_fract(x); }
 
 // ---- footprint box of a block for one angle ----------------------------------------
-// Returns (bu0, bv0, bw, bh): the detector samples [bu0, bu0+bw) x [bv0, bv0+bh)
+// Returns (bu0, bv0, bw | inner << 16, bh): the detector samples
+// [bu0, bu0+bw) x [bv0, bv0+bh)
 // that the block's bilinear taps can touch, widened by 2 samples on each side
 // for the rounding of the per-column values (the columns evaluate the same
 // float expressions as the corners); bw == 0: the block misses the detector;
@@ -217,7 +220,11 @@ __device__ int4 footprint(float c, float s, float x0, float x1, float y0, float 
     const int bw = bu1 - bu0 + 1, bh = bv1 - bv0 + 1;
     if (bw <= 0 || bh <= 0) return make_int4(0, 0, 0, 0);
     if (bw >= BOX_W || bh >= box_h) return make_int4(0, 0, -1, 0);
-    return make_int4(bu0, bv0, bw, bh);
+    // bit 16 of the width: every sample row of the block lies inside
+    // [0.02, Nv-1.02] (the per-column rows differ from the corner bounds by
+    // rounding only), so the gathers need no range test
+    const bool inner = vlo >= 0.02f && vhi <= (float)(Nv - 1) - 0.02f;
+    return make_int4(bu0, bv0, bw | (inner ? 0x10000 : 0), bh);
 }
 
 // ---- LDS gathers of one angle for LB slices of a thread's z-run ----------------------
@@ -257,12 +264,12 @@ __device__ __forceinline__ void gather_lds(float *acc, const float *az, const fl
 
 // FP16 pairs in LDS: word (r, c) holds samples c and c+1 of row r; one
 // ds_read2 fetches both rows of the 2x2 neighbourhood; the lerp along v runs
-// in packed FP16, the weights and the sum in FP32.
-typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+// in packed FP16; the two column weights (FP16, normalised to at most U^2) and
+// the two lerped columns meet in one v_dot2_f32_f16, summed in FP32.
 template <bool FAST, int LB>
 __device__ __forceinline__ void gather_pairs(float *acc, const float *az, const float *box,
-                                             float Uv, float cz, int colb, float wu0,
-                                             float wu1, float vmax)
+                                             float Uv, float cz, int colb, half2_t wuh,
+                                             float vmax)
 {
     const char *lds = (const char *)box;
     half2_t c[LB];
@@ -283,22 +290,27 @@ __device__ __forceinline__ void gather_pairs(float *acc, const float *az, const 
         c[k] = f * (w1 - w0) + w0;
         okk[k] = ok;
     }
+    // the three-address VOP3P form (the builtin selects the two-address
+    // v_dot2c form, whose tied accumulator costs register copies here)
 #pragma unroll
-    for (int k = 0; k < LB; ++k)
-        acc[k] = fmaf(okk[k] ? wu1 : 0.f, (float)c[k].y,
-                      fmaf(okk[k] ? wu0 : 0.f, (float)c[k].x, acc[k]));
+    for (int k = 0; k < LB; ++k) {
+        const half2_t w = okk[k] ? wuh : half2_t{0, 0};
+        float r;
+        asm("v_dot2_f32_f16 %0, %1, %2, %3" : "=v"(r) : "v"(c[k]), "v"(w), "v"(acc[k]));
+        acc[k] = r;
+    }
 }
 
 // all ZR slices of a thread for one staged angle, LB gathers at a time
 template <bool FAST, bool PAIRS, int ZR, int LB>
 __device__ __forceinline__ void gather_run(float (&acc)[ZR], const float (&az)[ZR],
                                            const float *box, float Uv, float cz, int colb,
-                                           float wu0, float wu1, float vmax)
+                                           float wu0, float wu1, half2_t wuh, float vmax)
 {
 #pragma unroll
     for (int k0 = 0; k0 < ZR; k0 += LB) {
         if constexpr (PAIRS)
-            gather_pairs<FAST, LB>(acc + k0, az + k0, box, Uv, cz, colb, wu0, wu1, vmax);
+            gather_pairs<FAST, LB>(acc + k0, az + k0, box, Uv, cz, colb, wuh, vmax);
         else
             gather_lds<FAST, LB>(acc + k0, az + k0, box, Uv, cz, colb, wu0, wu1, vmax);
         __builtin_amdgcn_sched_barrier(0);
@@ -356,9 +368,15 @@ struct BoxStage {
             if constexpr (ST::PAIRS) {
                 const uint2 w = raw[i];
                 const unsigned nx = __shfl_down(w.x, 1, 16);
-                if (r <= bh)
-                    *(uint4 *)dst = make_uint4(w.x, (w.x >> 16) | (w.y << 16), w.y,
-                                               (w.y >> 16) | (nx << 16));
+                if (r <= bh) {
+                    // four 32-bit stores (ds_write2_b32 pairs take any registers;
+                    // one 128-bit store would need the words copied together)
+                    unsigned *d = (unsigned *)dst;
+                    d[0] = w.x;
+                    d[1] = (w.x >> 16) | (w.y << 16);
+                    d[2] = w.y;
+                    d[3] = (w.y >> 16) | (nx << 16);
+                }
             } else if constexpr (ST::BYTES == 4) {
                 if (r <= bh) *(float4 *)dst = raw[i];
             } else {
@@ -420,6 +438,20 @@ __global__ void __launch_bounds__(256) bp_kernel(BPArgs<typename Store<S>::R> a)
 #pragma unroll
     for (int k = 0; k < ZR; ++k) acc[k] = 0;
 
+    // FP16 pairs carry the column weights in FP16: they are normalised by the
+    // largest projection scale (rounded up to a power of two, g), so they stay
+    // within [0, U^2], and the sums are multiplied by g at the end
+    R wnorm = 1, wback = 1;
+    if constexpr (ST::PAIRS) {
+        float m = 0.f;
+        for (int i = lane; i < a.nang; i += 64) m = fmaxf(m, (float)a.scales[i]);
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+        const float g = m > 0.f ? exp2f(ceilf(log2f(m))) : 1.f;
+        wnorm = 1.f / g;
+        wback = g;
+    }
+
     // every wave of the workgroup joins the staging barriers, so the decision to
     // skip the angle loop is workgroup-wide
     if (__syncthreads_or(inside)) {
@@ -445,6 +477,9 @@ __global__ void __launch_bounds__(256) bp_kernel(BPArgs<typename Store<S>::R> a)
                         footprint(a.ang[2 * bb], a.ang[2 * bb + 1], cx0, cx1, cy0, cy1, azlo,
                                   azhi, a.cX, a.cZ, a.D, a.Nu, a.Nv, BOX_H);
             };
+            // the gathers of an angle without a staged box read the box with zero
+            // weight, so it must never hold non-finite garbage
+            for (int i = threadIdx.x; i < BOX_P * BOX_H * NBUF; i += 256) boxmem[i] = 0.f;
             boxes_of_group(0);
             boxes_of_group(1);
             __syncthreads();
@@ -491,35 +526,41 @@ __global__ void __launch_bounds__(256) bp_kernel(BPArgs<typename Store<S>::R> a)
                     stage.issue(st ? proj + a.angle_bytes : a.stack, st ? nx.x : 0,
                                 st ? nx.y : 0, st ? nx.w : -1, a.row_bytes, a.Nv);
                 }
-                if (bx.z <= 0) {              // misses the detector, or pass 2
-                    need_hbm = need_hbm || bx.z < 0;
-                    continue;
-                }
-                const int bu0 = bx.x, bv0 = bx.y;
+                // angles without a staged box (missing the detector, or left to
+                // pass 2) run the gathers with zero weight on the previous box:
+                // one code path through the loop keeps the accumulators in place
+                need_hbm = need_hbm || bx.z < 0;
+                const bool live = bx.z > 0;
+                const int bu0 = live ? bx.x : 0, bv0 = live ? bx.y : 0;
 
                 const R c = a.ang[2 * b], s = a.ang[2 * b + 1];
-                // the reference's S, T, U and u with nvcc's contractions
+                // the reference's S, T, U and u with nvcc's contractions; the
+                // FP16 path, whose error is set by its storage, takes U from the
+                // hardware reciprocal instead of the IEEE quotient
                 const R Sx = rfma(aX, c, aY * s);
                 const R T = rfma(aY, c, -(aX * s));
-                const R U = a.D / (a.D - Sx);
+                R U;
+                if constexpr (ST::PAIRS)
+                    U = a.D * __builtin_amdgcn_rcpf(a.D - Sx);
+                else
+                    U = a.D / (a.D - Sx);
                 const R u = rfma(T, U, a.cX);
-                const bool act = inside && u >= (R)0 && u <= a.umax;
+                const bool act = live && inside && u >= (R)0 && u <= a.umax;
                 const R fu = rfract(u);
                 const int iu = act ? (int)u : 0;            // u >= 0: trunc = floor
-                const R W = act ? U * U * a.scales[b] : (R)0;
+                const R W = act ? U * U * (a.scales[b] * wnorm) : (R)0;
                 const R wu1 = W * fu, wu0 = W - wu1;
+                half2_t wuh = {0, 0};
+                if constexpr (ST::PAIRS) wuh = half2_t{(_Float16)wu0, (_Float16)wu1};
                 // LDS byte offset of sample (row iv, column iu) is
                 // iv*4*BOX_P + colb; inactive lanes sit at v = bv0, offset 0
                 const int colb = (act ? (iu - bu0) * 4 : 0) - bv0 * (4 * BOX_P);
                 const R Uv = act ? U : (R)0;
                 const R cz = act ? a.cZ : (R)bv0;
-                const R v_lo = rfma(az[0], Uv, cz), v_hi = rfma(az[ZR - 1], Uv, cz);
-                const R vmin = v_lo < v_hi ? v_lo : v_hi, vmax = v_lo < v_hi ? v_hi : v_lo;
-                const bool fast =
-                    __all(!act || (vmin >= (R)0.01 && vmax <= a.vmax - (R)0.01));
+                const bool fast = !live || (bx.z & 0x10000);
                 if (fast) {
                     gather_run<true, ST::PAIRS, ZR, LB>(acc, az, box, Uv, cz, colb, wu0, wu1,
-                                                        a.vmax);
+                                                        wuh, a.vmax);
                 } else {
                     // an opaque copy keeps the compiler from hoisting the
                     // position arithmetic shared by both runs out of the
@@ -527,7 +568,7 @@ __global__ void __launch_bounds__(256) bp_kernel(BPArgs<typename Store<S>::R> a)
                     R Uv2 = Uv, cz2 = cz;
                     asm volatile("" : "+v"(Uv2), "+v"(cz2));
                     gather_run<false, ST::PAIRS, ZR, LB>(acc, az, box, Uv2, cz2, colb, wu0, wu1,
-                                                         a.vmax);
+                                                         wuh, a.vmax);
                 }
             }
         }
@@ -550,7 +591,7 @@ __global__ void __launch_bounds__(256) bp_kernel(BPArgs<typename Store<S>::R> a)
                 const bool act = inside && u >= (R)0 && u <= a.umax;
                 const R fu = rfract(u);
                 const int iu = act ? (int)u : 0;
-                const R W = act ? U * U * a.scales[b] : (R)0;
+                const R W = act ? U * U * (a.scales[b] * wnorm) : (R)0;
                 const R wu1 = W * fu, wu0 = W - wu1;
                 const R Uv = act ? U : (R)0;
                 const R cz = act ? a.cZ : (R)0;
@@ -582,7 +623,7 @@ __global__ void __launch_bounds__(256) bp_kernel(BPArgs<typename Store<S>::R> a)
     for (int k = 0; k < ZR; ++k) {
         const int z = z0 + k;
         if (z < a.Nz) {
-            const R r = inside ? acc[k] * a.out_scale : (R)0;
+            const R r = inside ? acc[k] * (a.out_scale * wback) : (R)0;
             if (a.accumulate)
                 out[(size_t)z * plane] += r;
             else

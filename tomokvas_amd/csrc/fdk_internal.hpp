@@ -100,7 +100,7 @@ int launch_backproject(int store_dtype, const BPLaunch &a, hipStream_t st);
 int bp_zrun(int store_dtype);
 int sample_bytes_of(int store_dtype);   // bytes of one stored sample
 int stack_pitch(int Nu);                // row pitch of the stored stack, samples
-constexpr int STACK_TAIL_ROWS = 96;     // spare rows after the last projection (>= box rows)
+constexpr int STACK_TAIL_ROWS = 128;    // spare rows after the last projection (>= box rows)
 
 struct FPLaunch {           // adjoint of the backprojector
     const void *vol;        // [Nz][Ny][Nx]
