@@ -50,7 +50,7 @@
 #define TKV_BP_ZR_F16 32
 #endif
 #ifndef TKV_BP_ZR_BF16
-#define TKV_BP_ZR_BF16 24
+#define TKV_BP_ZR_BF16 32
 #endif
 #ifndef TKV_BP_LB
 #define TKV_BP_LB 4

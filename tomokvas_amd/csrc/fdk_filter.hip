@@ -7,8 +7,8 @@
 // two real rows travel as the real and imaginary parts of one complex
 // sequence (the spectrum is Hermitian-preserving, so the two filtered rows come
 // back as the real and imaginary parts), zero-padded to p = 2*nextpow2(Nu),
-// forward Stockham FFT (radix-4 passes, one radix-2 pass when log2 p is odd;
-// Van Loan, Computational Frameworks for the FFT, 1992), bin-wise product with
+// forward Stockham FFT (radix-16 passes and at most one radix-2/4/8 pass; Van
+// Loan, Computational Frameworks for the FFT, 1992), bin-wise product with
 // the spectrum, inverse FFT without 1/p (the backprojector applies it, as the
 // reference does at FBPG/kernel.cu:575, :113), crop to Nu.  HBM traffic: one
 // read and one write of each detector sample; everything else stays in LDS.
@@ -17,9 +17,10 @@
 #include <hip/hip_bf16.h>
 #include <hip/hip_fp16.h>
 
-// waves per SIMD the row filter is compiled for (2: no spills at p = 4096)
+// waves per SIMD the row filter is compiled for (3: 168 VGPRs, a few spills
+// at p = 4096, yet faster than 2 waves without spills)
 #ifndef TKV_FILTER_WAVES
-#define TKV_FILTER_WAVES 2
+#define TKV_FILTER_WAVES 3
 #endif
 
 namespace tkv {
