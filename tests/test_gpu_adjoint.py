@@ -44,6 +44,47 @@ def test_dot_product_adjoint_fp64(torch, geo):
     assert O.rel_rmse(FPx.cpu().numpy(), ref) <= 1e-12
 
 
+# geometries of the FP32 adjoint checks; "wide" has voxels of 4 detector
+# samples, so many blocks splat straight into HBM instead of through LDS
+ADJ_GEOMS = {
+    "small": lambda: small(),
+    "odd": lambda: scan(37, 23, 7, 19, 23, 17, D=60.0, tau=0.1, scale=1.3, cX=17.2, cZ=11.7,
+                        rot=0.3, flips=(1, 0, 1)),
+    "wide": lambda: scan(96, 96, 20, 40, 40, 40, D=300.0, tau=0.1, scale=4.0),
+}
+
+
+@pytest.mark.parametrize("geo", list(ADJ_GEOMS))
+def test_forward_project_fp32_matches_fp64(torch, geo):
+    g = ADJ_GEOMS[geo]()
+    st = stages(g)
+    gen = torch.Generator(device="cuda").manual_seed(11)
+    x = torch.randn(st.vol_shape, dtype=torch.float64, device="cuda", generator=gen)
+    y = torch.randn(st.sino_shape(), dtype=torch.float64, device="cuda", generator=gen)
+    f64 = st.forward_project(x).cpu().numpy()
+    f32 = st.forward_project(x.float()).double().cpu().numpy()
+    # the CPU FP32 mirror samples at bit-identical positions; only the order of
+    # the atomic sums differs
+    ref32 = O.forward_project(ogeom(g), x.float().cpu().numpy())
+    e32 = O.rel_rmse(f32, ref32)
+    e64 = O.rel_rmse(f32, f64)
+    # FP32 adjoint vs FP32 backprojector: <BP y, x> = <y, BP^T x>
+    stack, scales = st.pack(y.float(), _lib.F32)
+    bpy = st.backproject(stack, _lib.F32, scales).double()
+    lhs = float((bpy * x).sum())
+    rhs = float((y * torch.from_numpy(f32).cuda()).sum())
+    rel = abs(lhs - rhs) / abs(lhs)
+    say(f"  FP32 adjoint {geo}: vs FP32 mirror {e32:.2e}, vs FP64 {e64:.2e}, "
+        f"dot-product rel {rel:.2e}")
+    assert e32 <= 1e-5 and rel <= 1e-5
+    if geo != "small":
+        # "small" has exact 45-degree angles: in FP32 cos == sin, so U == 1 and
+        # the outermost slices land exactly on the detector's first and last
+        # rows, which FP64 rounding puts just outside (a full-sample jump with
+        # random data); elsewhere FP32 and FP64 differ by position rounding
+        assert e64 <= 1e-5
+
+
 def test_full_operator_adjoint_fp64(torch):
     from fdk_autograd import fdk, ramlak_taps
     g = small()
@@ -179,3 +220,45 @@ def test_cfg4_full_size_step(torch):
     eh = O.rel_rmse(out[torch.float32][1], out[torch.float64][1])
     say(f"  cfg4 FP32 vs FP64 gradient: dp {ep:.2e}, dh {eh:.2e}")
     assert ep <= 1e-4 and eh <= 1e-4
+
+
+def test_cfg5_fp16_forward_chunked_backward(torch):
+    """cfg5 at full size: 1024^3 from 1200 x 1536^2, learnable 4096 taps.  One
+    step with the FP16-stored forward and 50-projection chunk recomputation in
+    the backward pass, against the FP32 forward with the whole stack at once;
+    loss = sum of squares over the central 256^3 crop of the volume."""
+    import time
+    from fdk_autograd import fdk, ramlak_taps
+    w = geometry.workloads()["cfg5"]
+    st = stages(w.geom)
+    assert st.p == 4096
+    sino = st.phantom(w.ellipsoids())
+    taps = torch.from_numpy(ramlak_taps(st).astype(np.float32)).cuda()
+    Nz, Ny, Nx = st.vol_shape
+    crop = (slice(Nz // 2 - 128, Nz // 2 + 128), slice(Ny // 2 - 128, Ny // 2 + 128),
+            slice(Nx // 2 - 128, Nx // 2 + 128))
+    out = {}
+    for name, storage, chunk in (("f16_chunk50", _lib.F16, 50), ("f32_whole", _lib.F32, None)):
+        torch.cuda.synchronize()
+        torch.cuda.reset_peak_memory_stats()
+        base = torch.cuda.memory_allocated()
+        t0 = time.perf_counter()
+        p = sino.clone().requires_grad_(True)
+        t = taps.clone().requires_grad_(True)
+        vol = fdk(p, t, st, storage=storage, chunk=chunk)
+        loss = (vol[crop] ** 2).sum()
+        loss.backward()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        peak = (torch.cuda.max_memory_allocated() - base) / 2**30
+        say(f"  cfg5 {name}: step {dt:.2f} s, peak HBM above the inputs {peak:.1f} GiB")
+        out[name] = (p.grad.double(), t.grad.double().cpu().numpy())
+        del p, t, vol, loss
+    gp16, gh16 = out["f16_chunk50"]
+    gp32, gh32 = out["f32_whole"]
+    ep = float(torch.sqrt(((gp16 - gp32) ** 2).sum() / (gp32 ** 2).sum()))
+    eh = O.rel_rmse(gh16, gh32)
+    say(f"  cfg5 gradient, FP16 chunked vs FP32 whole: dp {ep:.2e}, dh {eh:.2e}")
+    # the FP16 stack changes the forward volume by ~1e-4 relative, which the
+    # crop loss's gradient inherits
+    assert ep <= 1e-3 and eh <= 1e-3
