@@ -215,24 +215,32 @@ __device__ __forceinline__ void gather_pairs(float *acc, const float *az, const 
                                              float Uv, float cz, int colb, half2_t wuh,
                                              float vmax)
 {
+    static_assert(LB % 2 == 0, "fractions are converted in pairs");
     const char *lds = (const char *)box;
     half2_t c[LB];
     bool okk[LB];
 #pragma unroll
-    for (int k = 0; k < LB; ++k) {
-        const float v = fmaf(az[k], Uv, cz);
-        const float fv = __builtin_amdgcn_fractf(v);
-        bool ok = true;
-        int off = (int)v * (4 * BOX_P) + colb;
-        if (!FAST) {
-            ok = v >= 0.f && v <= vmax;
-            off = ok ? off : 0;
+    for (int k = 0; k < LB; k += 2) {
+        const float va = fmaf(az[k], Uv, cz), vb = fmaf(az[k + 1], Uv, cz);
+        // one conversion serves two slices; each lerp broadcasts its half
+        const half2_t fab = __builtin_bit_cast(
+            half2_t, __builtin_amdgcn_cvt_pkrtz(__builtin_amdgcn_fractf(va),
+                                                __builtin_amdgcn_fractf(vb)));
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const float v = j ? vb : va;
+            bool ok = true;
+            int off = (int)v * (4 * BOX_P) + colb;
+            if (!FAST) {
+                ok = v >= 0.f && v <= vmax;
+                off = ok ? off : 0;
+            }
+            const half2_t w0 = *(const half2_t *)(lds + off);
+            const half2_t w1 = *(const half2_t *)(lds + off + 4 * BOX_P);
+            const half2_t f = j ? half2_t{fab.y, fab.y} : half2_t{fab.x, fab.x};
+            c[k + j] = f * (w1 - w0) + w0;
+            okk[k + j] = ok;
         }
-        const half2_t w0 = *(const half2_t *)(lds + off);
-        const half2_t w1 = *(const half2_t *)(lds + off + 4 * BOX_P);
-        const half2_t f = __builtin_bit_cast(half2_t, __builtin_amdgcn_cvt_pkrtz(fv, fv));
-        c[k] = f * (w1 - w0) + w0;
-        okk[k] = ok;
     }
     // the three-address VOP3P form (the builtin selects the two-address
     // v_dot2c form, whose tied accumulator costs register copies here)
@@ -282,7 +290,8 @@ struct BoxStage {
         // buffer loads: a scalar descriptor at the box origin whose range ends
         // with the last row inside the detector, so rows past it read as zero
         // (raw buffer range check), and one 32-bit offset per row
-        const char *base = proj + (size_t)bv0 * row_bytes + (size_t)bu0 * ST::BYTES;
+        // (a projection is below 2 GiB, checked at launch: 32-bit offsets)
+        const char *base = proj + ((uint32_t)bv0 * row_bytes + (uint32_t)bu0 * ST::BYTES);
         const int rows = min(bh, Nv - 1 - bv0);   // last row inside the detector
         const int nrec = (rows + 1) * (int)row_bytes - bu0 * ST::BYTES;
         const __amdgpu_buffer_rsrc_t rsrc =
@@ -302,32 +311,33 @@ struct BoxStage {
         }
     }
 
+    // rows are written in blocks of 16 (one per load) while the block starts
+    // inside the box (bh uniform: a scalar branch); rows past the box in the
+    // last block were loaded as zeros (outside the descriptor range), so the
+    // tile stays finite
     __device__ __forceinline__ void commit(float *box, int bh) const
     {
         const int r0 = threadIdx.x >> 4, col = (threadIdx.x & 15) * 4;
 #pragma unroll
         for (int i = 0; i < PER; ++i) {
+            if (16 * i > bh) break;
             const int r = r0 + 16 * i;
             float *dst = box + r * BOX_P + col;
             if constexpr (ST::PAIRS) {
                 const uint2 w = raw[i];
-                const unsigned nx = __shfl_down(w.x, 1, 16);
-                if (r <= bh) {
-                    // four 32-bit stores (ds_write2_b32 pairs take any registers;
-                    // one 128-bit store would need the words copied together)
-                    unsigned *d = (unsigned *)dst;
-                    d[0] = w.x;
-                    d[1] = (w.x >> 16) | (w.y << 16);
-                    d[2] = w.y;
-                    d[3] = (w.y >> 16) | (nx << 16);
-                }
+                // the next lane's first two samples (DPP row shift by one
+                // within the 16 lanes of a row; the last lane's 4th word, column
+                // 63, is never read)
+                const unsigned nx = (unsigned)__builtin_amdgcn_mov_dpp((int)w.x, 0x101, 0xf, 0xf,
+                                                                       true);
+                *(uint4 *)dst = make_uint4(w.x, __builtin_amdgcn_alignbyte(w.y, w.x, 2), w.y,
+                                           __builtin_amdgcn_alignbyte(nx, w.y, 2));
             } else if constexpr (ST::BYTES == 4) {
-                if (r <= bh) *(float4 *)dst = raw[i];
+                *(float4 *)dst = raw[i];
             } else {
                 const uint2 w = raw[i];
-                if (r <= bh)
-                    *(float4 *)dst = make_float4(ST::half(w.x, 0), ST::half(w.x, 1),
-                                                 ST::half(w.y, 0), ST::half(w.y, 1));
+                *(float4 *)dst = make_float4(ST::half(w.x, 0), ST::half(w.x, 1),
+                                             ST::half(w.y, 0), ST::half(w.y, 1));
             }
         }
     }
@@ -609,6 +619,8 @@ static int launch_bp_t(const BPLaunch &L, hipStream_t st)
     a.stack = (const char *)L.stack;
     a.row_bytes = (uint32_t)((size_t)L.pitch * Store<S>::BYTES);
     a.angle_bytes = (size_t)a.row_bytes * L.Nv;
+    if (a.angle_bytes > 0x7fffffffULL)
+        return fail(FDK_ERR_UNSUPPORTED, "backproject: projection of 2 GiB or more");
     a.ang = (const R *)L.ang;
     a.scales = (const R *)L.scales;
     a.ax = (const R *)L.ax;
