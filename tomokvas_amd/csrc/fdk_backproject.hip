@@ -62,11 +62,6 @@
 #ifndef TKV_BP_F16_PAIRS
 #define TKV_BP_F16_PAIRS 1
 #endif
-//   TKV_BP_DBUF                    two LDS boxes used alternately (one barrier
-//                                  per angle instead of two)
-#ifndef TKV_BP_DBUF
-#define TKV_BP_DBUF 0
-#endif
 
 namespace tkv {
 
@@ -360,9 +355,14 @@ __global__ void __launch_bounds__(256) bp_kernel(BPArgs<typename Store<S>::R> a)
     using ST = Store<S>;
     constexpr bool STAGED = ST::STAGED;
     constexpr int BOX_H = box_rows<ZR>();
-    constexpr int NBUF = TKV_BP_DBUF ? 2 : 1;
-    __shared__ float boxmem[STAGED ? NBUF * BOX_P * BOX_H : 1];
-    __shared__ int4 boxes[STAGED ? 128 : 1];   // footprints of two groups of 64 angles
+    __shared__ float boxmem[STAGED ? BOX_P * BOX_H : 1];
+    // per group of 64 angles, in two slots (groups g and g+1): the footprints,
+    // the group's angle order (inner angles from the front, edge angles from
+    // the back) and the two counts; and whether any angle is left to pass 2
+    __shared__ int4 boxes[STAGED ? 128 : 1];
+    __shared__ unsigned char order[STAGED ? 128 : 1];
+    __shared__ int counts[4];
+    __shared__ int hbm_flag;
 
     // XCD-aware block order: hardware dispatches workgroup i to XCD i % 8; the
     // remap gives each XCD a contiguous range of logical blocks.
@@ -420,73 +420,63 @@ __global__ void __launch_bounds__(256) bp_kernel(BPArgs<typename Store<S>::R> a)
         const float cx0 = (float)a.ax[xa], cx1 = (float)a.ax[xb];
         const float cy0 = (float)a.ay[ya], cy1 = (float)a.ay[yb];
 
-        // pass 1: angles whose footprint fits the LDS tile
+        // pass 1: angles whose footprint fits the LDS tile.  Wave 0 evaluates
+        // a group's 64 footprints (one per lane) and sorts the staged angles
+        // into inner ones (every tap row inside the detector: gathers without
+        // range tests) and edge ones; each kind runs in a loop of its own, so
+        // the accumulators never pass a branch merge (which costs a copy of
+        // all of them per angle).  Angles missing the detector are dropped;
+        // those whose box exceeds the tile are left to pass 2.
         bool need_hbm = !STAGED;
         if constexpr (STAGED) {
-            // footprints of 64 angles, one per lane of wave 0, into slot (g & 1)
             auto boxes_of_group = [&](int g) {
-                const int bb = g * 64 + lane;
-                if (wave == 0 && bb < a.nang)
-                    boxes[(g & 1) * 64 + lane] =
-                        footprint(a.ang[2 * bb], a.ang[2 * bb + 1], cx0, cx1, cy0, cy1, azlo,
-                                  azhi, a.cX, a.cZ, a.D, a.Nu, a.Nv, BOX_H);
+                if (wave != 0) return;
+                const int bb = g * 64 + lane, slot = g & 1;
+                int4 q = make_int4(0, 0, 0, 0);
+                if (bb < a.nang)
+                    q = footprint(a.ang[2 * bb], a.ang[2 * bb + 1], cx0, cx1, cy0, cy1, azlo, azhi,
+                                  a.cX, a.cZ, a.D, a.Nu, a.Nv, BOX_H);
+                boxes[slot * 64 + lane] = q;
+                const bool inner = q.z > 0 && (q.z & 0x10000);
+                const bool edge = q.z > 0 && !(q.z & 0x10000);
+                const uint64_t mi = __ballot(inner), me = __ballot(edge);
+                const uint64_t mh = __ballot(q.z < 0);
+                // rank among the lanes below of the same kind
+                const int ri = __builtin_amdgcn_mbcnt_hi((unsigned)(mi >> 32),
+                                                         __builtin_amdgcn_mbcnt_lo((unsigned)mi, 0));
+                const int re = __builtin_amdgcn_mbcnt_hi((unsigned)(me >> 32),
+                                                         __builtin_amdgcn_mbcnt_lo((unsigned)me, 0));
+                if (inner) order[slot * 64 + ri] = (unsigned char)lane;
+                if (edge) order[slot * 64 + 63 - re] = (unsigned char)lane;
+                if (lane == 0) {
+                    counts[2 * slot] = __popcll(mi);
+                    counts[2 * slot + 1] = __popcll(me);
+                    if (mh) hbm_flag = 1;
+                }
             };
-            // the gathers of an angle without a staged box read the box with zero
-            // weight, so it must never hold non-finite garbage
-            for (int i = threadIdx.x; i < BOX_P * BOX_H * NBUF; i += 256) boxmem[i] = 0.f;
-            boxes_of_group(0);
-            boxes_of_group(1);
-            __syncthreads();
-            BoxStage<S, BOX_H> stage;
-            auto uniform_box = [&](int bb) {
-                const int4 q = boxes[bb & 127];
+            auto uniform_box = [&](int slot, int i) {
+                const int4 q = boxes[slot * 64 + i];
                 return make_int4(__builtin_amdgcn_readfirstlane(q.x),
                                  __builtin_amdgcn_readfirstlane(q.y),
                                  __builtin_amdgcn_readfirstlane(q.z),
                                  __builtin_amdgcn_readfirstlane(q.w));
             };
-            // The prefetch is issued for every angle (an unconditional issue
-            // keeps one register set live): angles without a staged box get an
-            // empty descriptor range (bh = -1), whose loads return zeros.
-            {
-                const int4 b0 = uniform_box(0);
-                stage.issue(a.stack, b0.z > 0 ? b0.x : 0, b0.z > 0 ? b0.y : 0,
-                            b0.z > 0 ? b0.w : -1, a.row_bytes, a.Nv);
-            }
-            const char *proj = a.stack;
-            int nstaged = 0;
-            for (int b = 0; b < a.nang; ++b, proj += a.angle_bytes) {
-                if ((b & 63) == 0 && b > 0) {
-                    // the slot of group b/64 - 1 is free once every wave is here
-                    __syncthreads();
-                    boxes_of_group((b >> 6) + 1);
-                    __syncthreads();
-                }
-                const int4 bx = uniform_box(b);
-                // with two boxes, staged angle s uses box s % 2: it was last
-                // read by staged angle s-2, which every wave finished before
-                // the barrier of staged angle s-1
-                float *box = boxmem + (NBUF == 2 ? (nstaged & 1) * BOX_P * BOX_H : 0);
-                if (bx.z > 0) {
-                    if (NBUF == 1) __syncthreads();   // every wave is done with the box
-                    stage.commit(box, bx.w);
-                    __syncthreads();
-                    ++nstaged;
-                }
-                {
-                    // the next angle's box is loaded while this one is gathered
-                    const int4 nx = uniform_box(b + 1);
-                    const bool st = b + 1 < a.nang && nx.z > 0;
-                    stage.issue(st ? proj + a.angle_bytes : a.stack, st ? nx.x : 0,
-                                st ? nx.y : 0, st ? nx.w : -1, a.row_bytes, a.Nv);
-                }
-                // angles without a staged box (missing the detector, or left to
-                // pass 2) run the gathers with zero weight on the previous box:
-                // one code path through the loop keeps the accumulators in place
-                need_hbm = need_hbm || bx.z < 0;
-                const bool live = bx.z > 0;
-                const int bu0 = live ? bx.x : 0, bv0 = live ? bx.y : 0;
-
+            auto uniform_int = [](int v) { return __builtin_amdgcn_readfirstlane(v); };
+            // the tile starts finite: rows of a box past its last 16-row block
+            // are never written
+            for (int i = threadIdx.x; i < BOX_P * BOX_H; i += 256) boxmem[i] = 0.f;
+            if (threadIdx.x == 0) hbm_flag = 0;
+            __syncthreads();
+            boxes_of_group(0);
+            boxes_of_group(1);
+            __syncthreads();
+            BoxStage<S, BOX_H> stage;
+            auto issue = [&](int b, int4 q) {
+                stage.issue(a.stack + (size_t)b * a.angle_bytes, q.x, q.y, q.w, a.row_bytes, a.Nv);
+            };
+            // column weights and LDS offsets of one staged angle, then its gathers
+            auto gather_angle = [&](auto fast_tag, int b, int4 bx) {
+                constexpr bool FAST = decltype(fast_tag)::value;
                 const R c = a.ang[2 * b], s = a.ang[2 * b + 1];
                 // the reference's S, T, U and u with nvcc's contractions; the
                 // FP16 path, whose error is set by its storage, takes U from the
@@ -499,7 +489,7 @@ __global__ void __launch_bounds__(256) bp_kernel(BPArgs<typename Store<S>::R> a)
                 else
                     U = a.D / (a.D - Sx);
                 const R u = rfma(T, U, a.cX);
-                const bool act = live && inside && u >= (R)0 && u <= a.umax;
+                const bool act = inside && u >= (R)0 && u <= a.umax;
                 const R fu = rfract(u);
                 const int iu = act ? (int)u : 0;            // u >= 0: trunc = floor
                 const R W = act ? U * U * (a.scales[b] * wnorm) : (R)0;
@@ -508,23 +498,69 @@ __global__ void __launch_bounds__(256) bp_kernel(BPArgs<typename Store<S>::R> a)
                 if constexpr (ST::PAIRS) wuh = half2_t{(_Float16)wu0, (_Float16)wu1};
                 // LDS byte offset of sample (row iv, column iu) is
                 // iv*4*BOX_P + colb; inactive lanes sit at v = bv0, offset 0
-                const int colb = (act ? (iu - bu0) * 4 : 0) - bv0 * (4 * BOX_P);
+                const int colb = (act ? (iu - bx.x) * 4 : 0) - bx.y * (4 * BOX_P);
                 const R Uv = act ? U : (R)0;
-                const R cz = act ? a.cZ : (R)bv0;
-                const bool fast = !live || (bx.z & 0x10000);
-                if (fast) {
-                    gather_run<true, ST::PAIRS, ZR, LB>(acc, az, box, Uv, cz, colb, wu0, wu1,
-                                                        wuh, a.vmax);
-                } else {
-                    // an opaque copy keeps the compiler from hoisting the
-                    // position arithmetic shared by both runs out of the
-                    // branch (which would hold all ZR positions live)
-                    R Uv2 = Uv, cz2 = cz;
-                    asm volatile("" : "+v"(Uv2), "+v"(cz2));
-                    gather_run<false, ST::PAIRS, ZR, LB>(acc, az, box, Uv2, cz2, colb, wu0, wu1,
-                                                         wuh, a.vmax);
+                const R cz = act ? a.cZ : (R)bx.y;
+                gather_run<FAST, ST::PAIRS, ZR, LB>(acc, az, boxmem, Uv, cz, colb, wu0, wu1, wuh,
+                                                    a.vmax);
+            };
+            using fast_t = std::integral_constant<bool, true>;
+            using edge_t = std::integral_constant<bool, false>;
+            const int ngroups = (a.nang + 63) >> 6;
+            int pf = -1;   // angle whose box the stage registers are loading
+            for (int g = 0; g < ngroups; ++g) {
+                const int slot = g & 1, g0 = g * 64;
+                if (g > 0) {
+                    // the slot of group g-1 is free once every wave is here
+                    __syncthreads();
+                    boxes_of_group(g + 1);
+                    __syncthreads();
+                }
+                // edge angles: loaded and gathered one at a time
+                const int ne = uniform_int(counts[2 * slot + 1]);
+                for (int j = 0; j < ne; ++j) {
+                    const int i = uniform_int(order[slot * 64 + 63 - j]);
+                    const int4 bx = uniform_box(slot, i);
+                    issue(g0 + i, bx);
+                    __syncthreads();   // every wave is done with the previous box
+                    stage.commit(boxmem, bx.w);
+                    __syncthreads();
+                    gather_angle(edge_t{}, g0 + i, bx);
+                }
+                // inner angles: the next one's box is loaded while one is gathered
+                const int ni = uniform_int(counts[2 * slot]);
+                if (ni > 0) {
+                    const int i = uniform_int(order[slot * 64]);
+                    if (pf != g0 + i) issue(g0 + i, uniform_box(slot, i));
+                }
+                pf = -1;
+                for (int j = 0; j < ni; ++j) {
+                    const int i = uniform_int(order[slot * 64 + j]);
+                    const int4 bx = uniform_box(slot, i);
+                    __syncthreads();   // every wave is done with the previous box
+                    stage.commit(boxmem, bx.w);
+                    __syncthreads();
+                    // the next inner angle: this group's, else the next group's
+                    // first when that group has no edge angles (whose loads
+                    // would take the stage registers)
+                    int nb = -1;
+                    int4 nq = make_int4(0, 0, 0, -1);   // none: an empty range, zeros
+                    if (j + 1 < ni) {
+                        const int k = uniform_int(order[slot * 64 + j + 1]);
+                        nb = g0 + k;
+                        nq = uniform_box(slot, k);
+                    } else if (g + 1 < ngroups && uniform_int(counts[2 * (slot ^ 1) + 1]) == 0 &&
+                               uniform_int(counts[2 * (slot ^ 1)]) > 0) {
+                        const int k = uniform_int(order[(slot ^ 1) * 64]);
+                        nb = g0 + 64 + k;
+                        nq = uniform_box(slot ^ 1, k);
+                    }
+                    issue(nb >= 0 ? nb : 0, nq);
+                    pf = nb;
+                    gather_angle(fast_t{}, g0 + i, bx);
                 }
             }
+            need_hbm = uniform_int(hbm_flag) != 0;
         }
 
         // pass 2: angles gathered straight from HBM -- all of them for FP64,
