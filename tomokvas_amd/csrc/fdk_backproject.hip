@@ -39,8 +39,9 @@
 //   TKV_BP_ZR_F32 / _F16 / _BF16   slices per thread (register z-run) per
 //                                  storage type (FP64 uses 8)
 //   TKV_BP_LB                      LDS gathers per lane issued together
-//   TKV_BP_BOXP                    LDS row pitch of the staged box in floats
-//                                  (>= 64, multiple of 4)
+//   TKV_BP_BOXP_F32 / _F16 / _BF16 LDS row pitch of the staged box in 32-bit
+//                                  words (>= 64, multiple of 4): rows of one
+//                                  column fall in different banks
 //   TKV_BP_F16_PAIRS               FP16 storage keeps FP16 sample pairs in LDS
 //                                  (one 32-bit LDS word per tap pair, lerp
 //                                  along v in packed FP16); 0: FP32 box
@@ -56,8 +57,14 @@
 #ifndef TKV_BP_LB
 #define TKV_BP_LB 4
 #endif
-#ifndef TKV_BP_BOXP
-#define TKV_BP_BOXP 64
+#ifndef TKV_BP_BOXP_F16
+#define TKV_BP_BOXP_F16 72
+#endif
+#ifndef TKV_BP_BOXP_F32
+#define TKV_BP_BOXP_F32 80
+#endif
+#ifndef TKV_BP_BOXP_BF16
+#define TKV_BP_BOXP_BF16 80
 #endif
 #ifndef TKV_BP_F16_PAIRS
 #define TKV_BP_F16_PAIRS 1
@@ -65,9 +72,7 @@
 
 namespace tkv {
 
-constexpr int BOX_P = TKV_BP_BOXP;   // LDS row pitch of the staged box, floats
 static_assert(box_rows<64>() <= STACK_TAIL_ROWS, "stored stack tail shorter than a box");
-static_assert(BOX_P >= BOX_W && BOX_P % 4 == 0, "box pitch");
 
 // Row pitch of the stored stack in samples: a multiple of 64 with at least
 // BOX_W + 4 zero samples after column Nu-1, so a staged box row that starts
@@ -103,6 +108,7 @@ struct Store<st_f32> {
     static constexpr int BYTES = 4;
     static constexpr bool STAGED = true;
     static constexpr int ZR = TKV_BP_ZR_F32;
+    static constexpr int P = TKV_BP_BOXP_F32;
     static constexpr bool PAIRS = false;
     static __device__ __forceinline__ float get(const char *p) { return *(const float *)p; }
     static __device__ __forceinline__ float half(unsigned, int) { return 0.f; }
@@ -113,6 +119,7 @@ struct Store<st_f16> {
     static constexpr int BYTES = 2;
     static constexpr bool STAGED = true;
     static constexpr int ZR = TKV_BP_ZR_F16;
+    static constexpr int P = TKV_BP_BOXP_F16;
     static constexpr bool PAIRS = TKV_BP_F16_PAIRS != 0;
     static __device__ __forceinline__ float get(const char *p)
     {
@@ -129,6 +136,7 @@ struct Store<st_bf16> {
     static constexpr int BYTES = 2;
     static constexpr bool STAGED = true;
     static constexpr int ZR = TKV_BP_ZR_BF16;
+    static constexpr int P = TKV_BP_BOXP_BF16;
     static constexpr bool PAIRS = false;
     static __device__ __forceinline__ float get(const char *p)
     {
@@ -145,6 +153,7 @@ struct Store<st_f64> {
     static constexpr int BYTES = 8;
     static constexpr bool STAGED = false;
     static constexpr int ZR = 8;
+    static constexpr int P = 64;   // unused: FP64 is gathered from HBM
     static constexpr bool PAIRS = false;
     static __device__ __forceinline__ double get(const char *p) { return *(const double *)p; }
     static __device__ __forceinline__ float half(unsigned, int) { return 0.f; }
@@ -170,7 +179,7 @@ __device__ __forceinline__ double rfract(double x) { return __builtin_amdgcn_fra
 // acc[k] += W * bilinear(box; u, v_k) with v_k = az[k]*Uv + cz; colb is the
 // byte offset of the thread's column minus that of the box's first row.
 // FAST: every v_k of every lane of the wave lies inside [0.01, Nv-1.01].
-template <bool FAST, int LB>
+template <bool FAST, int LB, int P>
 __device__ __forceinline__ void gather_lds(float *acc, const float *az, const float *box,
                                            float Uv, float cz, int colb, float wu0, float wu1,
                                            float vmax)
@@ -183,14 +192,14 @@ __device__ __forceinline__ void gather_lds(float *acc, const float *az, const fl
         const float v = fmaf(az[k], Uv, cz);
         const float fv = __builtin_amdgcn_fractf(v);
         bool ok = true;
-        int off = (int)v * (4 * BOX_P) + colb;
+        int off = (int)v * (4 * P) + colb;
         if (!FAST) {
             ok = v >= 0.f && v <= vmax;
             off = ok ? off : 0;
         }
         const float *p0 = (const float *)(lds + off);
         const float q00 = p0[0], q01 = p0[1];
-        const float q10 = p0[BOX_P], q11 = p0[BOX_P + 1];
+        const float q10 = p0[P], q11 = p0[P + 1];
         c0[k] = fmaf(fv, q10 - q00, q00);
         c1[k] = fmaf(fv, q11 - q01, q01);
         okk[k] = ok;
@@ -205,7 +214,7 @@ __device__ __forceinline__ void gather_lds(float *acc, const float *az, const fl
 // ds_read2 fetches both rows of the 2x2 neighbourhood; the lerp along v runs
 // in packed FP16; the two column weights (FP16, normalised to at most U^2) and
 // the two lerped columns meet in one v_dot2_f32_f16, summed in FP32.
-template <bool FAST, int LB>
+template <bool FAST, int LB, int P>
 __device__ __forceinline__ void gather_pairs(float *acc, const float *az, const float *box,
                                              float Uv, float cz, int colb, half2_t wuh,
                                              float vmax)
@@ -225,13 +234,13 @@ __device__ __forceinline__ void gather_pairs(float *acc, const float *az, const 
         for (int j = 0; j < 2; ++j) {
             const float v = j ? vb : va;
             bool ok = true;
-            int off = (int)v * (4 * BOX_P) + colb;
+            int off = (int)v * (4 * P) + colb;
             if (!FAST) {
                 ok = v >= 0.f && v <= vmax;
                 off = ok ? off : 0;
             }
             const half2_t w0 = *(const half2_t *)(lds + off);
-            const half2_t w1 = *(const half2_t *)(lds + off + 4 * BOX_P);
+            const half2_t w1 = *(const half2_t *)(lds + off + 4 * P);
             const half2_t f = j ? half2_t{fab.y, fab.y} : half2_t{fab.x, fab.x};
             c[k + j] = f * (w1 - w0) + w0;
             okk[k + j] = ok;
@@ -249,7 +258,7 @@ __device__ __forceinline__ void gather_pairs(float *acc, const float *az, const 
 }
 
 // all ZR slices of a thread for one staged angle, LB gathers at a time
-template <bool FAST, bool PAIRS, int ZR, int LB>
+template <bool FAST, bool PAIRS, int ZR, int LB, int P>
 __device__ __forceinline__ void gather_run(float (&acc)[ZR], const float (&az)[ZR],
                                            const float *box, float Uv, float cz, int colb,
                                            float wu0, float wu1, half2_t wuh, float vmax)
@@ -257,9 +266,9 @@ __device__ __forceinline__ void gather_run(float (&acc)[ZR], const float (&az)[Z
 #pragma unroll
     for (int k0 = 0; k0 < ZR; k0 += LB) {
         if constexpr (PAIRS)
-            gather_pairs<FAST, LB>(acc + k0, az + k0, box, Uv, cz, colb, wuh, vmax);
+            gather_pairs<FAST, LB, P>(acc + k0, az + k0, box, Uv, cz, colb, wuh, vmax);
         else
-            gather_lds<FAST, LB>(acc + k0, az + k0, box, Uv, cz, colb, wu0, wu1, vmax);
+            gather_lds<FAST, LB, P>(acc + k0, az + k0, box, Uv, cz, colb, wu0, wu1, vmax);
         __builtin_amdgcn_sched_barrier(0);
     }
 }
@@ -317,7 +326,7 @@ struct BoxStage {
         for (int i = 0; i < PER; ++i) {
             if (16 * i > bh) break;
             const int r = r0 + 16 * i;
-            float *dst = box + r * BOX_P + col;
+            float *dst = box + r * ST::P + col;
             if constexpr (ST::PAIRS) {
                 const uint2 w = raw[i];
                 // the next lane's first two samples (DPP row shift by one
@@ -355,7 +364,9 @@ __global__ void __launch_bounds__(256) bp_kernel(BPArgs<typename Store<S>::R> a)
     using ST = Store<S>;
     constexpr bool STAGED = ST::STAGED;
     constexpr int BOX_H = box_rows<ZR>();
-    __shared__ float boxmem[STAGED ? BOX_P * BOX_H : 1];
+    constexpr int P = ST::P;
+    static_assert(P >= BOX_W && P % 4 == 0, "box pitch");
+    __shared__ float boxmem[STAGED ? P * BOX_H : 1];
     // per group of 64 angles, in two slots (groups g and g+1): the footprints,
     // the group's angle order (inner angles from the front, edge angles from
     // the back) and the two counts; and whether any angle is left to pass 2
@@ -464,7 +475,7 @@ __global__ void __launch_bounds__(256) bp_kernel(BPArgs<typename Store<S>::R> a)
             auto uniform_int = [](int v) { return __builtin_amdgcn_readfirstlane(v); };
             // the tile starts finite: rows of a box past its last 16-row block
             // are never written
-            for (int i = threadIdx.x; i < BOX_P * BOX_H; i += 256) boxmem[i] = 0.f;
+            for (int i = threadIdx.x; i < P * BOX_H; i += 256) boxmem[i] = 0.f;
             if (threadIdx.x == 0) hbm_flag = 0;
             __syncthreads();
             boxes_of_group(0);
@@ -497,12 +508,12 @@ __global__ void __launch_bounds__(256) bp_kernel(BPArgs<typename Store<S>::R> a)
                 half2_t wuh = {0, 0};
                 if constexpr (ST::PAIRS) wuh = half2_t{(_Float16)wu0, (_Float16)wu1};
                 // LDS byte offset of sample (row iv, column iu) is
-                // iv*4*BOX_P + colb; inactive lanes sit at v = bv0, offset 0
-                const int colb = (act ? (iu - bx.x) * 4 : 0) - bx.y * (4 * BOX_P);
+                // iv*4*P + colb; inactive lanes sit at v = bv0, offset 0
+                const int colb = (act ? (iu - bx.x) * 4 : 0) - bx.y * (4 * P);
                 const R Uv = act ? U : (R)0;
                 const R cz = act ? a.cZ : (R)bx.y;
-                gather_run<FAST, ST::PAIRS, ZR, LB>(acc, az, boxmem, Uv, cz, colb, wu0, wu1, wuh,
-                                                    a.vmax);
+                gather_run<FAST, ST::PAIRS, ZR, LB, P>(acc, az, boxmem, Uv, cz, colb, wu0, wu1,
+                                                       wuh, a.vmax);
             };
             using fast_t = std::integral_constant<bool, true>;
             using edge_t = std::integral_constant<bool, false>;
