@@ -417,7 +417,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TKV_FI
     cpx<R> reg[E];
     // first pass inputs straight from HBM: two rows as (re, im), optional
     // fused cosine weight (one evaluation when both rows share a detector
-    // row), zero padding beyond Nu
+    // row), zero padding beyond Nu.  Nu <= P/2 (checked at launch), so the
+    // inputs m >= RFIRST/2 of every butterfly (n >= P/2) are zero at compile
+    // time and the first pass folds them away.
     const bool same_v = vA == vB;
 #pragma unroll
     for (int b = 0; b < E / PL::RFIRST; ++b) {
@@ -426,14 +428,16 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TKV_FI
         for (int m = 0; m < PL::RFIRST; ++m) {
             const int n = i + m * (P / PL::RFIRST);
             R a = 0, c = 0;
-            if (n < Nu) {
+            if (m < (PL::RFIRST + 1) / 2 && n < Nu) {
                 if (hasA) a = inA[n];
                 if (hasB) c = inB[n];
                 if (weight) {
-                    R wa = cos_weight<R>(n, vA, cX, cZ, D);
-                    R wb = same_v ? wa : cos_weight<R>(n, vB, cX, cZ, D);
+                    const R wa = cos_weight<R>(n, vA, cX, cZ, D);
                     a *= wa;
-                    c *= wb;
+                    if (same_v)
+                        c *= wa;
+                    else
+                        c *= cos_weight<R>(n, vB, cX, cZ, D);
                 }
             }
             reg[b * PL::RFIRST + m] = {a, c};
@@ -484,11 +488,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TKV_FI
     R mA = 0, mB = 0;
     R *outA = out + (hasA ? rA : 0) * Nu;
     R *outB = out + (hasB ? rB : 0) * Nu;
+    // (outputs m >= RLAST/2, n >= P/2 >= Nu, are cropped: their arithmetic in
+    // the last pass is dead and removed at compile time)
 #pragma unroll
     for (int b = 0; b < E / PL::RLAST; ++b) {
         const int i = t + b * PL::TPT;
 #pragma unroll
-        for (int m = 0; m < PL::RLAST; ++m) {
+        for (int m = 0; m < (PL::RLAST + 1) / 2; ++m) {
             const int n = i + m * (P / PL::RLAST);
             if (n < Nu) {
                 cpx<R> z = reg[b * PL::RLAST + DI::pos(m)];
@@ -520,6 +526,7 @@ static int launch_filter_p(const FilterLaunch &a, hipStream_t st)
     long long blocks = (pairs + PL::NT - 1) / PL::NT;
     if (blocks == 0) return FDK_OK;
     if (blocks > 0x7fffffffLL) return fail(FDK_ERR_UNSUPPORTED, "filter: too many rows");
+    if (2 * a.Nu > P) return fail(FDK_ERR_INVALID_ARGUMENT, "filter: rows longer than p/2");
     size_t lds = PL::NPASS > 1 || PL::RFIRST != PL::RLAST
                      ? sizeof(cpx<R>) * (size_t)PL::NT * Pad<R, P>::SIZE
                      : 0;
