@@ -246,6 +246,13 @@ def main():
         "compulsory_hbm_bytes": g.Nang * g.Nv * g.Nu * s_bytes + 4 * g.Nx * g.Ny * g.Nz,
         "hbm_traffic_rate_gbs": traffic / t_bp / 1e9 if traffic else None,
     }
+    # the L2 figure measured on this device (2 MiB, resident in every XCD's L2)
+    l2_gbs = C.c_double(0.0)
+    if L.fdk_probe_l2_read(2 << 20, C.byref(l2_gbs)) == 0 and l2_gbs.value > 0:
+        roofline["l2_measured"] = l2_gbs.value
+        roofline["frac_l2_measured"] = achieved / l2_gbs.value
+    else:
+        log(f"L2 probe failed: {_lib.last_error()}")
     if valu_per_update:
         # the gathers are served from LDS: the kernel is bound by VALU issue
         t_valu = upd * valu_per_update / VALU_LANE_OPS

@@ -167,6 +167,14 @@ int fdk_phantom_project(fdk_ctx *ctx, const double *ells, int n, int ang0, int n
 /* Poisson(I0*exp(-p)) counts, then -ln(max(n,1)/I0), in place (Philox4x32-10) */
 int fdk_poisson_log(float *d_p, size_t n, double I0, uint64_t seed, void *stream);
 
+/* ---- measurement aid (no reference counterpart) -------------------------------------- */
+/* L2-resident streaming-read bandwidth of the current device, in GB/s: every wave
+ * reads 1 KiB blocks of a zeroed buffer of footprint_bytes (a power of two in
+ * [64 KiB, 64 MiB]; 2 MiB stays in each XCD's L2) at pseudo-random block indices;
+ * best of three timed launches after one warm-up; synchronous on the null stream.
+ * The backprojector's gather roofline is priced against this figure. */
+int fdk_probe_l2_read(size_t footprint_bytes, double *gbs);
+
 #ifdef __cplusplus
 }
 #endif

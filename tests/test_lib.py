@@ -89,3 +89,12 @@ def test_phantom_tables():
     e = phantoms.random_ellipsoids(20251202, 64, 2048.0)
     assert e.shape == (64, 19)
     assert np.array_equal(e, phantoms.random_ellipsoids(20251202, 64, 2048.0))
+
+
+def test_l2_probe_rejects_bad_footprints():
+    # argument checks run before any HIP call
+    lib = _lib.load()
+    gbs = C.c_double(0.0)
+    for fp in (0, 1000, 3 << 20, 128 << 20):
+        assert lib.fdk_probe_l2_read(fp, C.byref(gbs)) == _lib.FDK_ERR_INVALID_ARGUMENT
+    assert lib.fdk_probe_l2_read(2 << 20, None) == _lib.FDK_ERR_INVALID_ARGUMENT

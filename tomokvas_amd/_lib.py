@@ -63,6 +63,7 @@ _SIGS = {
     "fdk_compare": (_i, [_vp, _vp, C.c_size_t, _vp, _vp]),
     "fdk_phantom_project": (_i, [_vp, _vp, _i, _i, _i, _vp, _vp]),
     "fdk_poisson_log": (_i, [_vp, C.c_size_t, C.c_double, C.c_uint64, _vp]),
+    "fdk_probe_l2_read": (_i, [C.c_size_t, C.POINTER(C.c_double)]),
 }
 
 _lib = None

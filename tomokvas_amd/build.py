@@ -28,6 +28,7 @@ SOURCES = [
     "fdk_backproject.hip",
     "fdk_adjoint.hip",
     "fdk_phantom.hip",
+    "fdk_probe.hip",
 ]
 
 # -packed-fp32-ops: never emit v_pk_{add,mul,fma}_f32.
