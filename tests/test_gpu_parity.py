@@ -151,7 +151,8 @@ def test_weight_kernel_matches_mirror(torch):
                        rtol=1e-15, atol=0)
 
 
-@pytest.mark.parametrize("Nu", [1, 2, 3, 5, 9, 17, 37, 64, 100, 200, 300, 600, 1024, 1536, 2048])
+@pytest.mark.parametrize("Nu", [1, 2, 3, 5, 9, 17, 37, 64, 100, 200, 300, 600, 1024, 1536, 2048,
+                                3000])
 def test_filter_kernel_all_lengths(torch, Nu):
     g = scan(Nu, 3, 5, 8, 8, 8, D=max(2500.0, Nu))
     for filt in (0, 1):
@@ -194,6 +195,30 @@ def test_filter_fused_weight_and_user_spectrum(torch):
     lhs = float((Fa * b).sum())
     rhs = float((a * Ftb).sum())
     assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+
+
+@pytest.mark.parametrize("Nu", [37, 1536])
+def test_filter_fp32_fused_weight_complex_spectrum(torch, Nu):
+    # FP32 rows (radix-8 passes over 512 threads at p = 4096): fused cosine
+    # weight, a complex user spectrum, and its conjugate for the adjoint
+    g = scan(Nu, 6, 9, 8, 8, 8, D=max(2500.0, Nu))
+    rng = np.random.default_rng(40 + Nu)
+    h = rng.standard_normal(g.p)
+    st = stages(g, taps=h)
+    x = rng.standard_normal((g.Nv, g.Nang, g.Nu))
+    re, im = O.spectrum_complex(h)
+    ref = O.filter_rows_complex(ogeom(g), O.weight(ogeom(g), x, f64=True), re, im, f64=True)
+    d = torch.from_numpy(x.astype(np.float32)).cuda()
+    st.filter(d, weight=True)
+    e = O.rel_rmse(d.cpu().numpy(), ref)
+    say(f"  FP32 fused filter Nu={Nu}: rel RMSE vs FP64 {e:.2e}")
+    assert e <= 2e-6
+    a = rng.standard_normal((g.Nv, g.Nang, g.Nu)).astype(np.float32)
+    b = rng.standard_normal((g.Nv, g.Nang, g.Nu)).astype(np.float32)
+    Fa = st.filter(torch.from_numpy(a).cuda()).cpu().numpy().astype(np.float64)
+    Ftb = st.filter(torch.from_numpy(b).cuda(), adjoint=True).cpu().numpy().astype(np.float64)
+    lhs, rhs = float((Fa * b).sum()), float((a * Ftb).sum())
+    assert abs(lhs - rhs) <= 1e-5 * (np.abs(Fa * b).sum())
 
 
 # "wide": voxels of 4 detector samples, so many block footprints exceed the

@@ -17,10 +17,20 @@
 #include <hip/hip_bf16.h>
 #include <hip/hip_fp16.h>
 
-// waves per SIMD the row filter is compiled for (3: 168 VGPRs, a few spills
-// at p = 4096, yet faster than 2 waves without spills)
+// waves per SIMD the radix-16 row filter is compiled for (4: 128 VGPRs
+// without spills at p = 4096; 3 and 5 measured 41 and 48 ms against 34 ms at
+// 1800 x 2048^2)
 #ifndef TKV_FILTER_WAVES
-#define TKV_FILTER_WAVES 3
+#define TKV_FILTER_WAVES 4
+#endif
+// 1: FP32 rows with p >= 2048 use radix-8 passes over 512 threads (8 values
+// per thread, 76 VGPRs, 6 waves per SIMD) -- one pass more per transform;
+// measured 38 ms against the radix-16 plan's 34 ms, so off by default
+#ifndef TKV_FILTER_R8
+#define TKV_FILTER_R8 0
+#endif
+#ifndef TKV_FILTER_R8_WAVES
+#define TKV_FILTER_R8_WAVES 6
 #endif
 
 namespace tkv {
@@ -194,22 +204,27 @@ __device__ __forceinline__ unsigned float_bits_abs(float m) { return __float_as_
 __device__ __forceinline__ unsigned float_bits_abs(double m) { return __float_as_uint((float)m); }
 
 // ---------------------------------------------------------------- radix-16 row filter
-// Mixed-radix Stockham plan: radix-16 passes, with the one radix-2/4/8 pass
-// that log2(p) mod 4 requires placed second, so the first and the last pass
-// are both radix 16 whenever p >= 256.  Every thread holds E = P/TPT complex
-// values (16; 32 for p = 8192) through all passes.
-template <int P>
+// Mixed-radix Stockham plan: passes of radix RMAX (16, or 8), with the one
+// smaller pass that log2(p) mod log2(RMAX) requires placed second, so the first
+// and the last pass are both of radix RMAX whenever p >= RMAX^2.  A transform
+// takes TPT = min(p/RMAX, BLOCK) threads; every thread holds E = P/TPT complex
+// values through all passes.
+template <int P_, int RMAX = 16, int BLOCK_ = 256>
 struct Plan {
+    static_assert(RMAX == 16 || RMAX == 8, "radix");
+    static constexpr int P = P_;
+    static constexpr int BLOCK = BLOCK_;
+    static constexpr int LR = RMAX == 16 ? 4 : 3;
     static constexpr int LOG2 = __builtin_ctz(P);
-    static constexpr int N16 = LOG2 / 4;
-    static constexpr int REM = 1 << (LOG2 % 4);
-    static constexpr int NPASS = N16 == 0 ? 1 : N16 + (REM > 1 ? 1 : 0);
-    static constexpr int TPT = P < 16 ? 1 : (P / 16 < 256 ? P / 16 : 256);
-    static constexpr int NT = 256 / TPT;
+    static constexpr int NR = LOG2 / LR;
+    static constexpr int REM = 1 << (LOG2 % LR);
+    static constexpr int NPASS = NR == 0 ? 1 : NR + (REM > 1 ? 1 : 0);
+    static constexpr int TPT = P < RMAX ? 1 : (P / RMAX < BLOCK ? P / RMAX : BLOCK);
+    static constexpr int NT = BLOCK / TPT;
     static constexpr int E = P / TPT;
     static constexpr int radix(int pass)
     {
-        return N16 == 0 ? REM : (pass == 0 ? 16 : (REM > 1 && pass == 1 ? REM : 16));
+        return NR == 0 ? REM : (pass == 0 ? RMAX : (REM > 1 && pass == 1 ? REM : RMAX));
     }
     static constexpr int RFIRST = radix(0);
     static constexpr int RLAST = radix(NPASS - 1);
@@ -217,9 +232,10 @@ struct Plan {
 
 // LDS index with one pad slot per 16 values (bank-conflict relief), when the
 // padded transforms still fit 64 KiB per workgroup
-template <typename R, int P>
+template <typename R, typename PL>
 struct Pad {
-    static constexpr bool ON = sizeof(cpx<R>) * (size_t)Plan<P>::NT * (P + P / 16) <= 65536;
+    static constexpr int P = PL::P;
+    static constexpr bool ON = sizeof(cpx<R>) * (size_t)PL::NT * (P + P / 16) <= 65536;
     static constexpr int SIZE = ON ? P + P / 16 : P;
     static __device__ __forceinline__ int at(int n) { return ON ? n + (n >> 4) : n; }
 };
@@ -318,21 +334,36 @@ struct DFT<R, 16, INV> {
 // one Stockham pass on the thread's registers: butterfly i = t + b*TPT holds
 // its inputs (indices i + m*P/RAD) in reg[b*RAD + m]; outputs belong at
 // ((i - k)*RAD + k) + m*pc with k = i mod pc
-template <typename R, int P, int RAD, bool INV>
+template <typename R, typename PL, int RAD, bool INV>
 __device__ __forceinline__ void pass_compute(cpx<R> *reg, const cpx<R> *__restrict__ tw, int t,
                                              int pc)
 {
-    using PL = Plan<P>;
+    constexpr int P = PL::P;
+    // the inverse passes read the forward passes' twiddles: an opaque copy of
+    // the table pointer per pass keeps the compiler from holding every pass's
+    // twiddles in registers from the forward transform to the inverse one
+    const cpx<R> *twp = tw;
+    asm volatile("" : "+s"(twp));
 #pragma unroll
     for (int b = 0; b < PL::E / RAD; ++b) {
         const int i = t + b * PL::TPT;
         const int k = i & (pc - 1);
         cpx<R> *x = reg + b * RAD;
         if (pc > 1) {
-            const int stride = P / (RAD * pc);
+            // twiddles W^(m k stride): the power-of-two multiples m = 1, 2, 4, 8
+            // from the table (exactly rounded), the others as products of two
+            // earlier ones (m = 2^j + r, r < 2^j: at most 3 roundings); a lane
+            // then reads 4 table entries instead of RAD-1 scattered ones
+            const int kk = k * (P / (RAD * pc));
+            cpx<R> wp[RAD];
 #pragma unroll
             for (int m = 1; m < RAD; ++m) {
-                cpx<R> w = tw[m * k * stride];
+                const int hi = 1 << (31 - __builtin_clz(m));   // largest power of 2 <= m
+                wp[m] = hi == m ? twp[m * kk] : cmul(wp[hi], wp[m - hi]);
+            }
+#pragma unroll
+            for (int m = 1; m < RAD; ++m) {
+                cpx<R> w = wp[m];
                 if (INV) w.y = -w.y;
                 x[m] = cmul(x[m], w);
             }
@@ -341,22 +372,21 @@ __device__ __forceinline__ void pass_compute(cpx<R> *reg, const cpx<R> *__restri
     }
 }
 
-template <typename R, int P, int RAD>
+template <typename R, typename PL, int RAD>
 __device__ __forceinline__ void pass_gather(cpx<R> *reg, const cpx<R> *s, int t)
 {
-    using PL = Plan<P>;
+    constexpr int P = PL::P;
 #pragma unroll
     for (int b = 0; b < PL::E / RAD; ++b) {
         const int i = t + b * PL::TPT;
 #pragma unroll
-        for (int m = 0; m < RAD; ++m) reg[b * RAD + m] = s[Pad<R, P>::at(i + m * (P / RAD))];
+        for (int m = 0; m < RAD; ++m) reg[b * RAD + m] = s[Pad<R, PL>::at(i + m * (P / RAD))];
     }
 }
 
-template <typename R, int P, int RAD>
+template <typename R, typename PL, int RAD>
 __device__ __forceinline__ void pass_scatter(const cpx<R> *reg, cpx<R> *s, int t, int pc)
 {
-    using PL = Plan<P>;
 #pragma unroll
     for (int b = 0; b < PL::E / RAD; ++b) {
         const int i = t + b * PL::TPT;
@@ -364,47 +394,46 @@ __device__ __forceinline__ void pass_scatter(const cpx<R> *reg, cpx<R> *s, int t
         const int j = (i - k) * RAD + k;
 #pragma unroll
         for (int m = 0; m < RAD; ++m)
-            s[Pad<R, P>::at(j + m * pc)] = reg[b * RAD + DFT<R, RAD, false>::pos(m)];
+            s[Pad<R, PL>::at(j + m * pc)] = reg[b * RAD + DFT<R, RAD, false>::pos(m)];
     }
 }
 
 // all passes of one transform; on entry reg holds the first pass's inputs, on
 // exit the outputs X[i + m*P/RLAST] of butterfly i = t + b*TPT in
 // reg[b*RLAST + DFT::pos(m)]
-template <typename R, int P, bool INV, int PASS>
+template <typename R, typename PL, bool INV, int PASS>
 struct Passes {
     static __device__ __forceinline__ void run(cpx<R> *reg, cpx<R> *s,
                                                const cpx<R> *__restrict__ tw, int t, int pc)
     {
-        using PL = Plan<P>;
         constexpr int RAD = PL::radix(PASS);
         if (PASS > 0) {
             __syncthreads();   // the previous pass's outputs are in LDS
-            pass_gather<R, P, RAD>(reg, s, t);
+            pass_gather<R, PL, RAD>(reg, s, t);
         }
-        pass_compute<R, P, RAD, INV>(reg, tw, t, pc);
+        pass_compute<R, PL, RAD, INV>(reg, tw, t, pc);
         if (PASS + 1 < PL::NPASS) {
             __syncthreads();   // every thread has read its inputs
-            pass_scatter<R, P, RAD>(reg, s, t, pc);
-            Passes<R, P, INV, (PASS + 1 < PL::NPASS ? PASS + 1 : PASS)>::run(reg, s, tw, t,
+            pass_scatter<R, PL, RAD>(reg, s, t, pc);
+            Passes<R, PL, INV, (PASS + 1 < PL::NPASS ? PASS + 1 : PASS)>::run(reg, s, tw, t,
                                                                            pc * RAD);
         }
     }
 };
 
-template <typename R, int P>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TKV_FILTER_WAVES)))
+template <typename R, typename PL, int WAVES>
+__global__ void __launch_bounds__(PL::BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES)))
     filter_kernel(const R *in, R *out, int Nu, int Nv, int nang, int weight, R cX, R cZ, R D,
                   const R *__restrict__ Hre, const R *__restrict__ Him, int conj,
                   const cpx<R> *__restrict__ tw, unsigned *__restrict__ maxabs)
 {
-    using PL = Plan<P>;
+    constexpr int P = PL::P;
     constexpr int E = PL::E;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int tid = threadIdx.x;
     const int f = tid / PL::TPT;          // transform within the workgroup
     const int t = tid - f * PL::TPT;      // thread within the transform
-    cpx<R> *s = reinterpret_cast<cpx<R> *>(smem_raw) + f * Pad<R, P>::SIZE;
+    cpx<R> *s = reinterpret_cast<cpx<R> *>(smem_raw) + f * Pad<R, PL>::SIZE;
     const long long rows = (long long)Nv * nang;
     const long long rA = 2 * ((long long)blockIdx.x * PL::NT + f);
     const long long rB = rA + 1;
@@ -443,7 +472,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TKV_FI
             reg[b * PL::RFIRST + m] = {a, c};
         }
     }
-    Passes<R, P, false, 0>::run(reg, s, tw, t, 1);
+    Passes<R, PL, false, 0>::run(reg, s, tw, t, 1);
 
     // bin-wise product with the spectrum (FiltrationKernel, kernel.cu:40-53), in
     // registers: X[i + m*P/RLAST] sits at reg[b*RLAST + pos(m)]; the product is
@@ -476,12 +505,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TKV_FI
             const int i = t + b * PL::TPT;
 #pragma unroll
             for (int m = 0; m < PL::RLAST; ++m)
-                s[Pad<R, P>::at(i + m * (P / PL::RLAST))] = reg[b * PL::RLAST + m];
+                s[Pad<R, PL>::at(i + m * (P / PL::RLAST))] = reg[b * PL::RLAST + m];
         }
         __syncthreads();
-        pass_gather<R, P, PL::RFIRST>(reg, s, t);
+        pass_gather<R, PL, PL::RFIRST>(reg, s, t);
     }
-    Passes<R, P, true, 0>::run(reg, s, tw, t, 1);
+    Passes<R, PL, true, 0>::run(reg, s, tw, t, 1);
 
     // crop to Nu straight to HBM, per-projection max |value|
     using DI = DFT<R, PL::RLAST, true>;
@@ -517,10 +546,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TKV_FI
     }
 }
 
-template <typename R, int P>
-static int launch_filter_p(const FilterLaunch &a, hipStream_t st)
+template <typename R, typename PL, int WAVES>
+static int launch_filter_plan(const FilterLaunch &a, hipStream_t st)
 {
-    using PL = Plan<P>;
+    constexpr int P = PL::P;
     long long rows = (long long)a.Nv * a.nang;
     long long pairs = (rows + 1) / 2;
     long long blocks = (pairs + PL::NT - 1) / PL::NT;
@@ -528,16 +557,28 @@ static int launch_filter_p(const FilterLaunch &a, hipStream_t st)
     if (blocks > 0x7fffffffLL) return fail(FDK_ERR_UNSUPPORTED, "filter: too many rows");
     if (2 * a.Nu > P) return fail(FDK_ERR_INVALID_ARGUMENT, "filter: rows longer than p/2");
     size_t lds = PL::NPASS > 1 || PL::RFIRST != PL::RLAST
-                     ? sizeof(cpx<R>) * (size_t)PL::NT * Pad<R, P>::SIZE
+                     ? sizeof(cpx<R>) * (size_t)PL::NT * Pad<R, PL>::SIZE
                      : 0;
-    auto kern = filter_kernel<R, P>;
+    auto kern = filter_kernel<R, PL, WAVES>;
     if (lds > 65536) return fail(FDK_ERR_UNSUPPORTED, "filter: transform does not fit LDS");
-    hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(256), lds, st, (const R *)a.in,
+    hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(PL::BLOCK), lds, st, (const R *)a.in,
                        (R *)a.out, a.Nu, a.Nv, a.nang, a.weight, (R)a.cX, (R)a.cZ, (R)a.D,
                        (const R *)a.Hre, (const R *)a.Him, a.conj, (const cpx<R> *)a.tw,
                        a.maxabs);
     TKV_HIP(hipGetLastError());
     return FDK_OK;
+}
+
+// long rows (p >= 2048) in FP32: radix-8 passes over 512 threads per
+// workgroup, 8 values per thread; otherwise radix 16 over 256 threads
+template <typename R, int P>
+static int launch_filter_p(const FilterLaunch &a, hipStream_t st)
+{
+    if constexpr (TKV_FILTER_R8 && sizeof(R) == 4 && P >= 2048)
+        return launch_filter_plan<R, Plan<P, 8, 512>, TKV_FILTER_R8_WAVES>(a, st);
+    else
+        // FP64 values take twice the registers: compiled for 3 waves
+        return launch_filter_plan<R, Plan<P>, sizeof(R) == 4 ? TKV_FILTER_WAVES : 3>(a, st);
 }
 
 template <typename R>
@@ -556,7 +597,7 @@ static int launch_filter_t(const FilterLaunch &a, hipStream_t st)
     case 2048: return launch_filter_p<R, 2048>(a, st);
     case 4096: return launch_filter_p<R, 4096>(a, st);
     case 8192:
-        if (sizeof(R) == 4) return launch_filter_p<R, 8192>(a, st);
+        if constexpr (sizeof(R) == 4) return launch_filter_p<R, 8192>(a, st);
         break;
     default: break;
     }
