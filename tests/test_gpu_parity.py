@@ -277,6 +277,44 @@ def test_fp16_scale_and_bf16_storage(torch, geo):
         assert e <= tol
 
 
+@pytest.mark.parametrize("geo", ["odd", "wide"])
+def test_pack_writes_the_stored_stack_exactly(torch, geo):
+    # [beta][v][pitch] rows: the samples (FP16: q * (1/scale) rounded once to
+    # nearest even -- the kernel's fused multiply-convert rounds the exact
+    # product, so the reference forms it exactly in FP64; BF16: rounded to
+    # nearest even), zeros past column Nu - 1; Nu = 37 takes the scalar loads,
+    # Nu = 96 the 16-byte ones
+    g = scan(**(ODD if geo == "odd" else WIDE))
+    st = stages(g)
+    rng = np.random.default_rng(8)
+    q = (rng.standard_normal((g.Nv, g.Nang, g.Nu)) * 100).astype(np.float32)
+    d = torch.from_numpy(q).cuda()
+    m = np.abs(q).max(axis=(0, 2)).astype(np.float32)
+    maxabs = torch.from_numpy(m.view(np.int32)).cuda()
+    qt = q.transpose(1, 0, 2)                                   # [beta][v][u]
+    for dt in (_lib.F32, _lib.F16, _lib.BF16):
+        stack, scales = st.pack(d, dt, maxabs)
+        raw = stack.cpu().numpy()
+        pitch = (g.Nu + 68 + 63) // 64 * 64
+        n = g.Nang * g.Nv * pitch
+        if dt == _lib.F32:
+            got = raw.view(np.float32)[:n].reshape(g.Nang, g.Nv, pitch)
+            want = qt
+        elif dt == _lib.F16:
+            sc = np.where(m > 0, m / np.float32(8192.0), np.float32(1)).astype(np.float32)
+            assert np.array_equal(scales.cpu().numpy(), sc)
+            inv = (np.float32(1) / sc).astype(np.float32)
+            want = (qt.astype(np.float64) * inv.astype(np.float64)[:, None, None]).astype(np.float16)
+            got = raw.view(np.float16)[:n].reshape(g.Nang, g.Nv, pitch)
+        else:
+            bits = qt.view(np.uint32).astype(np.uint64)
+            rne = ((bits + 0x7FFF + ((bits >> 16) & 1)) >> 16).astype(np.uint16)
+            want = rne
+            got = raw.view(np.uint16)[:n].reshape(g.Nang, g.Nv, pitch)
+        assert np.array_equal(got[:, :, :g.Nu], want), dt
+        assert not got[:, :, g.Nu:].any(), dt
+
+
 def test_phantom_projector_matches_oracle(torch, cfg1):
     g, sino = cfg1
     st = stages(g)

@@ -796,6 +796,18 @@ __global__ void __launch_bounds__(256)
     if (v == 0 && lane == 0) scales[beta] = sc;
     const R *src = filt + ((size_t)v * nang + beta) * Nu;
     typename Pack4<Out>::V *dst = stack + (size_t)row * (pitch / 4);
+    if constexpr (sizeof(R) == 4) {
+        if ((Nu & 3) == 0) {
+            // rows start 16-byte aligned: one 16-byte load per lane and group
+            const float4 *src4 = (const float4 *)src;
+            for (int g = lane; g < pitch / 4; g += 64) {
+                float4 q = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (4 * g < Nu) q = src4[g];
+                dst[g] = Pack4<Out>::make(q.x * inv, q.y * inv, q.z * inv, q.w * inv);
+            }
+            return;
+        }
+    }
     for (int g = lane; g < pitch / 4; g += 64) {
         const int u = 4 * g;
         R q[4];
