@@ -25,7 +25,7 @@ def fma(a, b, c):  # float32 fma via float64 (exact product)
 
 BOX_W, ALIGN = 64, 4
 # rows of the LDS box per z-run length (box_rows in the kernel)
-BOX_ROWS = {16: 64, 24: 80, 32: 96, 48: 128}
+BOX_ROWS = {16: 64, 24: 80, 32: 96, 48: 128, 64: 128}
 
 
 def footprint(c, s, x0, x1, y0, y1, azlo, azhi, cX, cZ, D, Nu, Nv, box_h):
@@ -139,8 +139,10 @@ def test_footprint_boxes_cover_every_tap(name, zr):
     make, ts, an = CASES[name]
     st = check(make(), ZR=zr, tile_stride=ts, ang_stride=an)
     assert st["taps"] > 0
-    if name.startswith("cfg"):
-        # the workloads never fall back to HBM gathers
+    if name.startswith("cfg") and (zr < 64 or name != "cfg3"):
+        # the workloads never fall back to HBM gathers (64-slice runs are the
+        # adjoint's, whose workloads have voxels of at most 1.5 samples; at
+        # cfg3's 2 samples a 64-slice box exceeds 128 rows and splats to HBM)
         assert st["hbm"] == 0, st
     if name == "wide":
         # this geometry exercises the HBM-gather pass next to the staged one

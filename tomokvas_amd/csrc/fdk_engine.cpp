@@ -672,6 +672,33 @@ int fdk_forward_project(fdk_ctx *c, int dtype, const void *d_vol, int ang0, int 
     L.D = c->g.D;
     L.out_scale = out_scale(c, dtype);
     L.maskR2 = 0.25 * (double)c->g.Nu * (double)c->g.Nu;
+    {
+        // per voxel-update the four splats weigh at most U^2 |x| together;
+        // columns inside the cylinder (|S| < r) have U <= D/(D-r), and the
+        // staged path never sees U >= 100 (its footprints need D-S > D/100).
+        // A sample of a 16 x 16-column, 64-slice workgroup's box receives at
+        // most one splat per column and slice:
+        //  - slices: those whose row v lies in [row-1, row+1), at most
+        //    1 + 2/spacing with the slice spacing scale * U >= scale*D/(D+r);
+        //  - columns: those whose u lies in [col-1, col+1).  Along a line of
+        //    the tile axis closer to the T direction (|cos| or |sin| >=
+        //    1/sqrt 2), u = cX + D T/(D-S) is monotone with
+        //    |du/d(step)| = pitch D |c'(D-S) + T s'| / (D-S)^2 >=
+        //    pitch D (D - 2r) / (sqrt 2 (D+r)^2) (|S|, |T| < r), so each of
+        //    the 16 lines holds at most 1 + 2/du of them (all 16 if D <= 2r).
+        const double r = std::sqrt(L.maskR2), D = c->g.D;
+        const double pitch = std::fabs((double)c->g.scale);
+        const double umax = D > r ? std::min(100.0, D / (D - r)) : 100.0;
+        const double umin = D / (D + r);
+        const double spacing = pitch * umin;
+        const double spr = std::min(64.0, 1.0 + std::floor(2.0 / std::max(spacing, 1e-9)));
+        const double du = D > 2.0 * r
+                              ? pitch * D * (D - 2.0 * r) / (std::sqrt(2.0) * (D + r) * (D + r))
+                              : 0.0;
+        const double per_line = du > 0.0 ? std::min(16.0, 1.0 + std::floor(2.0 / du)) : 16.0;
+        const double ncols = 16.0 * per_line;
+        L.fx_bound = ncols * spr * umax * umax;
+    }
     return tkv::launch_forward_project(f64 ? 8 : 4, L, pick(c, stream));
 }
 

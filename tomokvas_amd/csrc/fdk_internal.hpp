@@ -109,6 +109,10 @@ struct FPLaunch {           // adjoint of the backprojector
     int Nu, Nv, nang;
     int Nx, Ny, Nz;
     double cX, cZ, D, out_scale, maskR2;
+    // FP32 path: bound on the sum of the LDS contributions to one detector
+    // sample, per unit of the workgroup's largest |voxel| (fixes the scale of
+    // its fixed-point accumulation)
+    double fx_bound;
 };
 int launch_forward_project(int real_bytes, const FPLaunch &a, hipStream_t st);
 
