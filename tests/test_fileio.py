@@ -1,0 +1,216 @@
+"""The .CRA reader and .bin/.siz writer (include/tomokvas/fdk_io.h) -- host
+code, CPU only.
+
+Pinned by hand-derived known answers (header clamps), by the raw data itself
+(every sample encodes its file projection, row and column, so the loaded layout
+and the angle of every loaded projection can be checked against the
+acquisition angle), and by file contents written through the slab pipeline.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from tomokvas_amd import _lib
+from tomokvas_amd.fileio import BinSiz, CraCon
+
+KEYS = {  # the reference's key spelling, value column at 24 (mucat_file.cpp:15-28)
+    "raw": "FILE NAME:              ", "source": "SOURCE NAME:            ",
+    "spacing": "SAMPLE SPACING:         ", "distance": "DISTANCE:               ",
+    "dx": "DETECTOR X PIXELS:      ", "dz": "DETECTOR Z PIXELS:      ",
+    "n": "NUMBER OF PROJECTIONS:  ", "x": "X DIMENSION:            ",
+    "z": "Z DIMENSION:            ", "rot": "ROTATE ANGLE:           ",
+    "centre": "CENTRE:                 ", "blocks": "NUMBER OF BLOCKS:       ",
+    "scale": "SCALE:                  ",
+}
+
+
+def write_cra(folder, name, fields, raw=None, clockwise=False, crlf=False, centres=()):
+    lines = [KEYS[k] + str(v) for k, v in fields.items()]
+    lines += [KEYS["centre"] + str(c) for c in centres]
+    if clockwise:
+        lines.append("CLOCKWISE: yes")
+    lines.append("SOME OTHER LINE: ignored")
+    end = "\r\n" if crlf else "\n"
+    with open(os.path.join(folder, name + ".CRA"), "w", newline="") as f:
+        f.write(end.join(lines) + end)
+    if raw is not None:
+        raw.astype(np.float32).tofile(os.path.join(folder, fields["raw"]))
+
+
+def coded_raw(blocks, n, dz, dx):
+    """[block*n + j][v][u] = block*1e6 + j*1e3 + v*10 + u/4 (exact in float32
+    for these sizes) -- every sample names its origin."""
+    j = np.arange(blocks * n)[:, None, None]
+    v = np.arange(dz)[None, :, None]
+    u = np.arange(dx)[None, None, :]
+    b, jj = j // n, j % n
+    return (b * 1000000 + jj * 1000 + v * 10 + u / 4.0).astype(np.float32)
+
+
+def test_header_fields_defaults_and_clamps(tmp_path):
+    base = dict(raw="p.raw", source="tube A", spacing=100, distance=10, dx=1000, dz=800,
+                n=36, rot=12.5)
+    # distance 10 cm at 100 um -> 1000 px; x <= 2*int(1000*sin(atan(500/1000)) + 1)
+    # = 2*int(448.21) = 896; z <= int((1000 - x/2) / 1000 * 800) rounded down to even
+    cases = [
+        (dict(x=0, z=1000), 896, 440),      # (1000-448)*0.8 = 441.6 -> 440
+        (dict(x=2000, z=1000), 896, 440),
+        (dict(x=500, z=1000), 500, 600),    # (1000-250)*0.8 = 600
+        (dict(x=501, z=1000), 501, 600),    # 501/2 = 250 in integers (599.6 -> 598 in floats)
+        (dict(x=500, z=300), 500, 300),
+        (dict(x=500), 500, 1),              # no Z DIMENSION: 0 -> 1
+    ]
+    for extra, want_x, want_z in cases:
+        write_cra(str(tmp_path), "scan", {**base, **extra}, crlf=True, centres=(499.5, 501.25))
+        cra = CraCon(str(tmp_path), "scan")
+        assert cra.GetMetaData() == 0
+        m = cra.meta
+        assert m.raw_file.decode() == "p.raw" and m.source_name.decode() == "tube A"
+        assert (m.detector_x, m.detector_z, m.num_of_projections) == (1000, 800, 36)
+        assert (m.x_dimension, m.z_dimension) == (want_x, want_z), extra
+        assert m.num_of_blocks == 1 and m.scale == 1.0 and not m.clockwise
+        assert m.distance_px == np.float32(1000.0)
+        assert m.delta_fi == np.float32(2 * math.pi / 36)
+        assert list(m.centre_of_rotation[:m.num_centres]) == [499.5, 501.25]
+        assert cra.GetRecSize() == (want_x, want_x, want_z)
+        assert cra.GetSize() == (1000, 36, 800)
+        # one scan block: an out-of-range block selects block 0 (mucat_file.cpp:339-347)
+        assert cra.SetActiveBlock(1) == 0
+        assert cra.GetCentreOfRotation() == 499.5
+
+
+def test_missing_header_file_and_fields(tmp_path):
+    cra = CraCon(str(tmp_path), "none")
+    assert cra.GetMetaData() == _lib.FDK_ERR_INVALID_ARGUMENT
+    write_cra(str(tmp_path), "bad", dict(raw="p.raw", spacing=100, dx=10, dz=4, n=6))
+    assert CraCon(str(tmp_path), "bad").GetMetaData() == _lib.FDK_ERR_INVALID_ARGUMENT
+
+
+@pytest.mark.parametrize("clockwise", [False, True])
+@pytest.mark.parametrize("budget_proj", [1, 4, 5, 7, 100])
+def test_load_layout_and_angles(tmp_path, clockwise, budget_proj):
+    n, dz, dx, blocks = 7, 3, 5, 2
+    raw = coded_raw(blocks, n, dz, dx)
+    f = dict(raw="p.raw", spacing=50, distance=2, dx=dx, dz=dz, n=n, x=4, z=2, rot=30,
+             blocks=blocks)
+    write_cra(str(tmp_path), "s", f, raw=raw, clockwise=clockwise, centres=(2.0, 2.5))
+    cra = CraCon(str(tmp_path), "s")
+    assert cra.GetMetaData() == 0
+    cra.SetSinBlockMaxSize(budget_proj * dz * dx * 4)
+    nb = cra.CalculateSinBlocks()
+    assert nb == math.ceil(n / budget_proj)
+    dfi = float(np.float32(2 * math.pi / n))
+    for scan in range(blocks):
+        assert cra.SetActiveBlock(scan) == scan
+        assert cra.GetCentreOfRotation() == (2.0, 2.5)[scan]
+        seen = []
+        for k in range(nb):
+            cra.SetSinActiveBlock(k)
+            x, nk, z = cra.GetSinBlockSize()
+            assert (x, z) == (dx, dz) and nk == min(budget_proj, n - k * budget_proj)
+            d = cra.LoadData()                               # [v][a][u]
+            origin = np.floor(d / 1000.0).astype(np.int64)   # scan*1000 + file index j
+            assert (origin // 1000 == scan).all()
+            j = origin % 1000                                # [v][a][u] -> file projection
+            assert (j == j[:1, :, :1]).all()                 # one projection per a
+            jj = j[0, :, 0]
+            want = np.arange(k * budget_proj, k * budget_proj + nk)
+            assert list(jj) == list(want[::-1] if clockwise else want)
+            rest = d - (origin * 1000.0)
+            vv = np.arange(dz)[:, None, None]
+            uu = np.arange(dx)[None, None, :]
+            assert np.array_equal(rest, np.broadcast_to(vv * 10 + uu / 4.0, rest.shape))
+            # angle of loaded projection a = acquisition angle of file
+            # projection j, modulo 2 pi: +j*dfi anticlockwise, -j*dfi clockwise
+            # the clockwise formula adds n * dfi (2 pi up to dfi's float rounding)
+            tol = abs(n * dfi - 2 * math.pi) + 1e-12
+            rot = cra.GetRotRad()
+            for a, jf in enumerate(jj):
+                got = rot + a * dfi
+                acq = math.radians(30) + (-jf if clockwise else jf) * dfi
+                dd = (got - acq) % (2 * math.pi)
+                assert min(dd, 2 * math.pi - dd) <= tol, (k, a, jf)
+            seen += list(jj)
+        assert sorted(seen) == list(range(n))
+
+
+def test_split_rejects_budgets_that_drop_projections(tmp_path):
+    n, dz, dx = 10, 2, 3
+    write_cra(str(tmp_path), "s", dict(raw="p.raw", spacing=50, distance=2, dx=dx, dz=dz, n=n,
+                                       x=2, z=2), raw=coded_raw(1, n, dz, dx))
+    cra = CraCon(str(tmp_path), "s")
+    assert cra.GetMetaData() == 0
+    proj = dz * dx * 4
+    cra.SetSinBlockMaxSize(proj - 1)                 # below one projection
+    with pytest.raises(RuntimeError):
+        cra.CalculateSinBlocks()
+    # 80 bytes = 3 projections + 8: 240 / 80 = 3 blocks of 3, one projection dropped
+    # 84 bytes = 3 projections + 12: blocks of 3, 3 and a last one of 4 over budget
+    for budget in (n * proj // 3, 84):
+        cra.SetSinBlockMaxSize(budget)
+        with pytest.raises(RuntimeError):
+            cra.CalculateSinBlocks()
+    cra.SetSinBlockMaxSize(n * proj // 2 + 4)        # 5 projections + 4 bytes: 5 and 5
+    assert cra.CalculateSinBlocks() == 2
+    assert [cra.SetSinActiveBlock(k) == k and cra.GetSinBlockSize()[1] for k in (0, 1)] == [5, 5]
+    cra.SetSinBlockMaxSize(10 ** 12)                 # above the whole scan: one block
+    assert cra.CalculateSinBlocks() == 1
+    assert cra.GetSinBlockSize()[1] == n
+
+
+def test_short_raw_file_fails(tmp_path):
+    write_cra(str(tmp_path), "s", dict(raw="p.raw", spacing=50, distance=2, dx=4, dz=2, n=5,
+                                       x=2, z=2), raw=coded_raw(1, 3, 2, 4))
+    cra = CraCon(str(tmp_path), "s")
+    assert cra.GetMetaData() == 0
+    with pytest.raises(RuntimeError):
+        cra.LoadData()
+
+
+def read_bin(folder, name, shape):
+    return np.fromfile(os.path.join(folder, name + ".bin"), np.float32).reshape(shape)
+
+
+@pytest.mark.parametrize("z,spb", [(10, 4), (8, 4), (3, 8), (1, 1)])
+def test_bin_writes_slabs_and_collects(tmp_path, z, spb):
+    x, y, sb = 5, 3, 2
+    folder = str(tmp_path)
+    b = BinSiz(folder, "vol")
+    b.SetScanBlocksCount(sb)
+    b.SetImageSize(x, y, z)
+    b.SetSlicesPerBlock(spb)
+    assert os.path.getsize(os.path.join(folder, "vol.bin")) == sb * z * y * x * 4
+    nslab = math.ceil(z / spb)
+    rng = np.random.default_rng(z * 10 + spb)
+    want = np.zeros((sb, z, y, x), np.float32)
+    for scan in range(sb):
+        b.SetScanBlockOffset(scan)
+        for angle_block in range(3):             # three angle blocks, collected
+            part = rng.standard_normal((z, y, x)).astype(np.float32)
+            for n in range(nslab):
+                buf = b.GetBlock(n, collecting=angle_block > 0)
+                z0, nz = n * spb, min(spb, z - n * spb)
+                view = buf[: nz * y * x].reshape(nz, y, x)
+                if angle_block == 0:
+                    view[:] = part[z0:z0 + nz]
+                else:                            # the engine adds its partial
+                    view += part[z0:z0 + nz]
+            want[scan] += part
+        b.FlushBuffer()
+    b.close()
+    got = read_bin(folder, "vol", (sb, z, y, x))
+    assert np.allclose(got, want, rtol=1e-6, atol=1e-6)
+    assert open(os.path.join(folder, "vol.siz")).read() == f"{x} {y} {z * sb} 0"
+
+
+def test_bin_rejects_bad_slabs(tmp_path):
+    b = BinSiz(str(tmp_path), "v")
+    b.SetImageSize(2, 2, 4)
+    b.SetSlicesPerBlock(2)
+    with pytest.raises(RuntimeError):
+        b.GetBlock(2)
+    with pytest.raises(RuntimeError):
+        b.SetScanBlockOffset(1)
+    b.close()

@@ -1,4 +1,5 @@
-"""ctypes binding of libtomokvas_fdk.so (the C ABI in include/tomokvas/fdk.h).
+"""ctypes binding of libtomokvas_fdk.so (the C ABI in include/tomokvas/fdk.h and
+include/tomokvas/fdk_io.h).
 
 The library is built in place by ``python -m tomokvas_amd.build``.  Loading
 fails loudly when it is missing: there is no fallback implementation.
@@ -12,6 +13,7 @@ import re
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libtomokvas_fdk.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "tomokvas", "fdk.h")
+HEADERS = [HEADER, os.path.join(os.path.dirname(HERE), "include", "tomokvas", "fdk_io.h")]
 
 FDK_OK = 0
 FDK_ERR_INVALID_ARGUMENT = 1
@@ -64,7 +66,34 @@ _SIGS = {
     "fdk_phantom_project": (_i, [_vp, _vp, _i, _i, _i, _vp, _vp]),
     "fdk_poisson_log": (_i, [_vp, C.c_size_t, C.c_double, C.c_uint64, _vp]),
     "fdk_probe_l2_read": (_i, [C.c_size_t, C.POINTER(C.c_double)]),
+    # file formats (fdk_io.h)
+    "fdk_cra_open": (_i, [C.c_char_p, C.c_char_p, C.POINTER(_vp)]),
+    "fdk_cra_close": (None, [_vp]),
+    "fdk_cra_get_meta": (_i, [_vp, _vp]),
+    "fdk_cra_set_scan_block": (_i, [_vp, _i]),
+    "fdk_cra_split": (_i, [_vp, C.c_longlong, C.POINTER(_i)]),
+    "fdk_cra_set_angle_block": (_i, [_vp, _i]),
+    "fdk_cra_block_projections": (_i, [_vp, C.POINTER(_i)]),
+    "fdk_cra_rotation": (_i, [_vp, C.POINTER(C.c_double)]),
+    "fdk_cra_load": (_i, [_vp, _vp]),
+    "fdk_bin_open": (_i, [C.c_char_p, C.c_char_p, _i, _i, _i, _i, _i, C.POINTER(_vp)]),
+    "fdk_bin_set_scan_block": (_i, [_vp, _i]),
+    "fdk_bin_get_block": (C.POINTER(C.c_float), [_vp, _i, _i]),
+    "fdk_bin_flush": (_i, [_vp]),
+    "fdk_bin_close": (_i, [_vp]),
 }
+
+
+class CraMeta(C.Structure):
+    """fdk_cra_meta (include/tomokvas/fdk_io.h)."""
+    _fields_ = [("raw_file", C.c_char * 256), ("source_name", C.c_char * 256),
+                ("sample_spacing", C.c_double), ("distance", C.c_double),
+                ("detector_x", C.c_int), ("detector_z", C.c_int),
+                ("num_of_projections", C.c_int), ("num_of_blocks", C.c_int),
+                ("x_dimension", C.c_int), ("z_dimension", C.c_int),
+                ("rotate_angle", C.c_double), ("clockwise", C.c_int), ("num_centres", C.c_int),
+                ("centre_of_rotation", C.c_double * 64), ("scale", C.c_double),
+                ("distance_px", C.c_float), ("delta_fi", C.c_float)]
 
 _lib = None
 
@@ -76,8 +105,8 @@ class FDKError(RuntimeError):
 
 
 def header_functions() -> list:
-    """Names of every function the C ABI header declares."""
-    txt = open(HEADER).read()
+    """Names of every function the C ABI headers declare."""
+    txt = "".join(open(h).read() for h in HEADERS)
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
     names = re.findall(r"\b(fdk_[a-z0-9_]+)\s*\(", txt)
     return sorted(set(n for n in names if n != "fdk_get_block_fn"))

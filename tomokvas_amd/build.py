@@ -29,6 +29,7 @@ SOURCES = [
     "fdk_adjoint.hip",
     "fdk_phantom.hip",
     "fdk_probe.hip",
+    "fdk_io.cpp",
 ]
 
 # -packed-fp32-ops: never emit v_pk_{add,mul,fma}_f32.

@@ -157,6 +157,11 @@ class FilteredBackProjection:
 
     def CalculateReconstruction(self) -> int:
         sink = self._sink
+        native = getattr(sink, "native_get_block", None)
+        if native is not None:
+            # a C writer (fileio.BinSiz): the engine calls it directly
+            fn, user = native
+            return self._L.fdk_calculate_reconstruction(self._h, fn, user)
         keep = {}
 
         def get_block(user, n, collecting):
