@@ -46,7 +46,7 @@
 //                                  (one 32-bit LDS word per tap pair, lerp
 //                                  along v in packed FP16); 0: FP32 box
 #ifndef TKV_BP_ZR_F32
-#define TKV_BP_ZR_F32 16
+#define TKV_BP_ZR_F32 24
 #endif
 #ifndef TKV_BP_ZR_F16
 #define TKV_BP_ZR_F16 32
