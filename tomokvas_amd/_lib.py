@@ -13,7 +13,8 @@ import re
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libtomokvas_fdk.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "tomokvas", "fdk.h")
-HEADERS = [HEADER, os.path.join(os.path.dirname(HERE), "include", "tomokvas", "fdk_io.h")]
+HEADERS = [HEADER] + [os.path.join(os.path.dirname(HERE), "include", "tomokvas", h)
+                      for h in ("fdk_io.h", "fdk_driver.h")]
 
 FDK_OK = 0
 FDK_ERR_INVALID_ARGUMENT = 1
@@ -81,6 +82,11 @@ _SIGS = {
     "fdk_bin_get_block": (C.POINTER(C.c_float), [_vp, _i, _i]),
     "fdk_bin_flush": (_i, [_vp]),
     "fdk_bin_close": (_i, [_vp]),
+    # the reconstruction program (fdk_driver.h)
+    "fdk_setting_defaults": (_i, [_vp]),
+    "fdk_setting_load": (_i, [C.c_char_p, _vp]),
+    "fdk_setting_save": (_i, [C.c_char_p, _vp]),
+    "fdk_reconstruct_file": (_i, [C.c_char_p, _vp, _vp, _vp]),
 }
 
 
@@ -94,6 +100,30 @@ class CraMeta(C.Structure):
                 ("rotate_angle", C.c_double), ("clockwise", C.c_int), ("num_centres", C.c_int),
                 ("centre_of_rotation", C.c_double * 64), ("scale", C.c_double),
                 ("distance_px", C.c_float), ("delta_fi", C.c_float)]
+
+
+
+class Setting(C.Structure):
+    """fdk_setting (include/tomokvas/fdk_driver.h)."""
+    _fields_ = [("gpu_id", C.c_int), ("mem_limit_mib", C.c_int), ("alloc_limit_mib", C.c_int),
+                ("slices_per_block", C.c_int), ("x_direction", C.c_int),
+                ("y_direction", C.c_int), ("z_direction", C.c_int),
+                ("vertical_center", C.c_double), ("wait_for_enter", C.c_int)]
+
+
+class RunOptions(C.Structure):
+    """fdk_run_options (include/tomokvas/fdk_driver.h)."""
+    _fields_ = [("storage", C.c_int), ("filter", C.c_int), ("angle_block_bytes", C.c_longlong),
+                ("verbose", C.c_int)]
+
+
+class RunStats(C.Structure):
+    """fdk_run_stats (include/tomokvas/fdk_driver.h)."""
+    _fields_ = [("scan_blocks", C.c_int), ("angle_blocks", C.c_int),
+                ("seconds_total", C.c_double), ("seconds_load_wait", C.c_double),
+                ("seconds_engine", C.c_double), ("seconds_flush", C.c_double),
+                ("bytes_read", C.c_double), ("bytes_written", C.c_double)]
+
 
 _lib = None
 

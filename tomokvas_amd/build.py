@@ -1,4 +1,5 @@
-"""Build the gfx950 FDK library in place: tomokvas_amd/libtomokvas_fdk.so.
+"""Build the gfx950 FDK library in place: tomokvas_amd/libtomokvas_fdk.so, and
+the command line program tomokvas_amd/tomokvas-fbp linked against it.
 
 Every HIP/C++ source under tomokvas_amd/csrc is compiled by hipcc for gfx950
 (objects under tomokvas_amd/_build/) and linked into one shared library that
@@ -30,7 +31,11 @@ SOURCES = [
     "fdk_phantom.hip",
     "fdk_probe.hip",
     "fdk_io.cpp",
+    "fdk_driver.cpp",
 ]
+# the command line program, linked against the library (host code only)
+MAIN = "fdk_main.cpp"
+EXE = os.path.join(HERE, "tomokvas-fbp")
 
 # -packed-fp32-ops: never emit v_pk_{add,mul,fma}_f32.
 # -munsafe-fp-atomics: float/double atomic adds map to the L2 atomic units.
@@ -69,8 +74,15 @@ def build(verbose: bool = True) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-6000:]}")
     os.replace(tmp, LIB)
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"),
+           os.path.join(CSRC, MAIN), "-o", EXE + ".tmp", "-L", HERE, "-l:" + os.path.basename(LIB),
+           "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"{MAIN} failed:\n{r.stderr[-6000:]}")
+    os.replace(EXE + ".tmp", EXE)
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {EXE}")
     return LIB
 
 
