@@ -50,7 +50,56 @@ def hip():
     h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
     h.hipFree.argtypes = [C.c_void_p]
     h.hipDeviceSynchronize.argtypes = []
+    h.hipHostMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t, C.c_uint]
+    h.hipHostFree.argtypes = [C.c_void_p]
     return h
+
+
+def slab_pipeline(fbp, g, hipm, storage, steps):
+    """FDK with every slab handed to the host (page-locked memory, the writer's
+    case): the backprojection runs in chunks while the previous chunk's slabs
+    leave on a second stream.  Returns the stage times and the transfer time
+    left exposed after the last backprojection."""
+    L = _lib.load()
+    plane = g.Nx * g.Ny
+    nslab = -(-g.Nz // g.B)
+    host = C.c_void_p()
+    assert hipm.hipHostMalloc(C.byref(host), nslab * g.B * plane * 4, 0) == 0
+    base = host.value
+    cb = _lib.GET_BLOCK_FN(lambda user, n, collecting: base + n * g.B * plane * 4)
+    _lib.check(fbp.SetStorage(storage), "SetStorage")
+    _lib.check(L.fdk_calculate_reconstruction(fbp.handle, cb, None), "slab warmup")
+    walls, stages = [], []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        _lib.check(L.fdk_calculate_reconstruction(fbp.handle, cb, None), "slab step")
+        walls.append(time.perf_counter() - t0)
+        stages.append(fbp.stage_times_ms())
+    # the last slabs must hold the device volume
+    last = np.empty(plane, np.float32)
+    z = g.Nz - 1
+    assert hipm.hipMemcpy(last.ctypes.data, C.c_void_p(fbp.device_volume_ptr() + z * plane * 4),
+                          plane * 4, 2) == 0
+    got = np.ctypeslib.as_array(C.cast(C.c_void_p(base + z * plane * 4),
+                                       C.POINTER(C.c_float)), (plane,))
+    same = bool(np.array_equal(got, last))
+    # the same bytes in one device-to-host copy: what the slab transfers cost
+    # when they do not overlap the backprojection
+    vol_bytes = plane * g.Nz * 4
+    hipm.hipDeviceSynchronize()
+    t0 = time.perf_counter()
+    assert hipm.hipMemcpy(C.c_void_p(base), C.c_void_p(fbp.device_volume_ptr()), vol_bytes, 2) == 0
+    t_copy = time.perf_counter() - t0
+    hipm.hipHostFree(host)
+    st = np.array(stages).mean(axis=0)
+    chunk = C.c_int()
+    _lib.check(L.fdk_get_volume_chunk(fbp.handle, C.byref(chunk)), "chunk")
+    return {"storage": NAMES[storage], "host": "page-locked", "chunk_slices": chunk.value,
+            "filter_ms": float(st[0]), "pack_ms": float(st[1]),
+            "bp_ms": float(st[2]), "exposed_transfer_ms": float(st[3]),
+            "wall_ms": 1e3 * float(np.mean(walls)), "volume_bytes": vol_bytes,
+            "d2h_gbs": vol_bytes / t_copy / 1e9, "serial_transfer_ms": 1e3 * t_copy,
+            "slabs_equal_device_volume": same}
 
 
 def read_profile(workload: str, storage: str):
@@ -229,6 +278,16 @@ def main():
         log(f"{NAMES[st]}: filter {filt_ms:.2f} ms, pack {pack_ms:.2f} ms, BP {bp_ms:.2f} ms "
             f"({res[NAMES[st]]['gups']:.1f} GUPS), FDK {fdk_ms:.2f} ms, vs f32 {cmp}")
     hipm.hipFree(ref)
+    pipe_storage = _lib.F16 if "f16" in res else order[0]
+    pipe = slab_pipeline(fbp, g, hipm, pipe_storage, args.steps)
+    r0 = res[NAMES[pipe_storage]]
+    pipe["fdk_without_handout_ms"] = r0["fdk_ms"]
+    pipe["fdk_with_handout_ms"] = pipe["filter_ms"] + pipe["pack_ms"] + pipe["bp_ms"] + \
+        pipe["exposed_transfer_ms"]
+    log(f"slab hand-out ({pipe['storage']}, chunks of {pipe['chunk_slices']} slices): BP "
+        f"{pipe['bp_ms']:.2f} ms, exposed transfers {pipe['exposed_transfer_ms']:.2f} ms, "
+        f"wall {pipe['wall_ms']:.1f} ms, slabs equal device volume: "
+        f"{pipe['slabs_equal_device_volume']}")
 
     head = "f16" if "f16" in res else NAMES[order[0]]
     r = res[head]
@@ -275,6 +334,7 @@ def main():
         "rel_rmse": {f"{k}_vs_f32_full_volume": v["rel_rmse_vs_f32"] for k, v in res.items()
                      if v["rel_rmse_vs_f32"] is not None},
         "roofline": roofline,
+        "slab_pipeline": pipe,
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:

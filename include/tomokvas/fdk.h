@@ -46,6 +46,10 @@ extern "C" {
 #define FDK_FILTER_SHEPPLOGAN  1   /* apodised extension                              */
 #define FDK_FILTER_USER        2   /* p spatial taps h, H = DFT_p(h)                  */
 
+/* where the reconstructed volume lives during fdk_calculate_reconstruction */
+#define FDK_VOLUME_RESIDENT    0   /* whole volume in HBM, readable with fdk_device_volume */
+#define FDK_VOLUME_STREAMED    1   /* two chunks in HBM; slabs leave through the callback  */
+
 typedef struct fdk_ctx fdk_ctx;
 
 /* Host slab sink, replacing `float* bin_siz::GetBlock(int n, bool collecting)`
@@ -90,6 +94,14 @@ int fdk_set_scale_xyz(fdk_ctx *ctx, float scale);                  /* SetScaleXY
  * storage type of the filtered stack (default FDK_F32) */
 int fdk_set_filter(fdk_ctx *ctx, int kind, const double *taps, int ntaps);
 int fdk_set_storage(fdk_ctx *ctx, int dtype);
+/* extension: volume residency (default FDK_VOLUME_RESIDENT).  With a slab
+ * callback both modes backproject the volume in chunks of whole slabs and move
+ * chunk k's slabs to the host on a second stream while chunk k+1 is computed;
+ * FDK_VOLUME_STREAMED keeps only two chunks in HBM, so the volume may exceed
+ * it (kernel.cu:819-888 computes one slab at a time without overlap). */
+int fdk_set_volume_mode(fdk_ctx *ctx, int mode);
+/* slices per chunk chosen by fdk_initialize (a multiple of the block size) */
+int fdk_get_volume_chunk(fdk_ctx *ctx, int *slices);
 
 /* ---- engine run ---------------------------------------------------------------------- */
 /* Initialize() (kernel.cu:324-428): allocate, upload the borrowed sinogram,

@@ -1,0 +1,110 @@
+"""The slab pipeline of fdk_calculate_reconstruction: the volume backprojected
+in chunks of whole slabs while the previous chunk's slabs go to the host on a
+second stream; resident (whole volume in HBM) and streamed (two chunks in HBM)
+volumes.
+
+Every voxel is computed by the same kernel code from the same inputs whether
+the volume is backprojected whole or in chunks, so the slabs handed out must
+equal the one-launch volume bit for bit.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle as O
+from tomokvas_amd import _lib
+from tomokvas_amd.engine import VolumeSink
+
+from gpu_util import engine, ogeom, say, scan
+
+pytestmark = pytest.mark.gpu
+
+_hip = None
+
+
+def d2h(ptr, shape):
+    global _hip
+    if _hip is None:
+        _hip = C.CDLL("libamdhip64.so")
+    out = np.empty(shape, np.float32)
+    assert _hip.hipMemcpy(C.c_void_p(out.ctypes.data), C.c_void_p(ptr), C.c_size_t(out.nbytes),
+                          2) == 0
+    return out
+
+
+def one_launch(g, s, storage):
+    """No sink: one backprojection over the whole volume, read from HBM."""
+    fbp = engine(g, s, None, storage=storage)
+    _lib.check(fbp.Initialize(), "Initialize")
+    _lib.check(fbp.CalculateReconstruction(), "CalculateReconstruction")
+    vol = d2h(fbp.device_volume_ptr(), (g.Nz, g.Ny, g.Nx))
+    fbp.close()
+    return vol
+
+
+def piped(g, s, storage, mode, sink=None, collecting=False):
+    sink = sink or VolumeSink(g.Nx, g.Ny, g.Nz, g.B)
+    fbp = engine(g, s, sink, storage=storage, collecting=collecting)
+    _lib.check(fbp.SetVolumeMode(mode), "SetVolumeMode")
+    _lib.check(fbp.Initialize(), "Initialize")
+    chunk = fbp.GetVolumeChunk()
+    _lib.check(fbp.CalculateReconstruction(), "CalculateReconstruction")
+    times = fbp.stage_times_ms()
+    streamed_ptr = fbp.device_volume_ptr()
+    fbp.close()
+    return sink, chunk, times, streamed_ptr
+
+
+# (Nz, B): one chunk; two chunks with a ragged last slab; four chunks (the
+# streamed ring is reused twice)
+CASES = [(33, 10), (40, 6), (80, 4)]
+
+
+@pytest.mark.parametrize("storage", [_lib.F32, _lib.F16])
+@pytest.mark.parametrize("Nz,B", CASES)
+def test_chunked_slabs_equal_one_launch(Nz, B, storage):
+    g = scan(48, 40, 30, 48, 44, Nz, B=B)
+    rng = np.random.default_rng(Nz * 100 + B)
+    s = rng.standard_normal((g.Nv, g.Nang, g.Nu)).astype(np.float32)
+    want = one_launch(g, s, storage)
+    for mode in (_lib.VOLUME_RESIDENT, _lib.VOLUME_STREAMED):
+        say(f"pipeline Nz={Nz} B={B} storage={storage} mode={mode}")
+        sink, chunk, times, ptr = piped(g, s, storage, mode)
+        nslab = -(-Nz // B)
+        assert chunk % B == 0 and chunk >= B
+        assert [c[0] for c in sink.calls] == list(range(nslab))
+        assert not any(c[1] for c in sink.calls)
+        assert (ptr == 0) == (mode == _lib.VOLUME_STREAMED)
+        np.testing.assert_array_equal(sink.volume, want)
+        say(f"  chunk {chunk} slices, {-(-Nz // chunk)} chunks; BP {times[2]:.3f} ms, "
+            f"exposed transfers {times[3]:.3f} ms")
+
+
+def test_streamed_collecting_two_angle_blocks():
+    # two angle blocks accumulated through the sink with a streamed volume of
+    # four chunks, against the resident one-pass volume
+    N = 40
+    g = scan(48, 40, N, 48, 48, 80, B=4)
+    s = O.project_ellipsoids(ogeom(g), O.shepp_logan(24.0)).astype(np.float32)
+    full = one_launch(g, s, _lib.F32)
+    sink = VolumeSink(g.Nx, g.Ny, g.Nz, g.B)
+    for blk in range(2):
+        gb = scan(48, 40, N // 2, 48, 48, 80, B=4, rot=blk * (N // 2) * g.dbeta)
+        gb.dbeta = g.dbeta
+        part = np.ascontiguousarray(s[:, blk * N // 2:(blk + 1) * N // 2, :])
+        piped(gb, part, _lib.F32, _lib.VOLUME_STREAMED, sink=sink, collecting=bool(blk))
+    assert sink.calls[-1] == (19, True)
+    e = O.rel_rmse(sink.volume, full)
+    say(f"  streamed two-block accumulation vs one pass rel RMSE {e:.3e}")
+    assert e <= 1e-5
+
+
+def test_streamed_volume_needs_a_sink():
+    g = scan(32, 32, 8, 32, 32, 16, B=8)
+    fbp = engine(g, np.zeros((32, 8, 32), np.float32), None)
+    _lib.check(fbp.SetVolumeMode(_lib.VOLUME_STREAMED))
+    _lib.check(fbp.Initialize())
+    assert fbp.CalculateReconstruction() == _lib.FDK_ERR_INVALID_ARGUMENT
+    assert fbp.SetVolumeMode(7) == _lib.FDK_ERR_INVALID_ARGUMENT
+    fbp.close()

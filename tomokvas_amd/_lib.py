@@ -26,6 +26,7 @@ FDK_ERR_UNSUPPORTED = 6
 
 F32, F16, BF16, F64 = 0, 1, 2, 3
 FILTER_RAMLAK, FILTER_SHEPPLOGAN, FILTER_USER = 0, 1, 2
+VOLUME_RESIDENT, VOLUME_STREAMED = 0, 1
 
 GET_BLOCK_FN = C.CFUNCTYPE(C.c_void_p, C.c_void_p, C.c_int, C.c_int)
 
@@ -67,6 +68,8 @@ _SIGS = {
     "fdk_phantom_project": (_i, [_vp, _vp, _i, _i, _i, _vp, _vp]),
     "fdk_poisson_log": (_i, [_vp, C.c_size_t, C.c_double, C.c_uint64, _vp]),
     "fdk_probe_l2_read": (_i, [C.c_size_t, C.POINTER(C.c_double)]),
+    "fdk_set_volume_mode": (_i, [_vp, _i]),
+    "fdk_get_volume_chunk": (_i, [_vp, C.POINTER(_i)]),
     # file formats (fdk_io.h)
     "fdk_cra_open": (_i, [C.c_char_p, C.c_char_p, C.POINTER(_vp)]),
     "fdk_cra_close": (None, [_vp]),

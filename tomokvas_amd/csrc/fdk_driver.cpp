@@ -8,6 +8,8 @@
 //    down to whole projections so that no projection is left out of a block;
 //  * the next angle block is read from disk on a loader thread while the engine
 //    works on the current one (two host buffers);
+//  * the volume is streamed: two chunks in HBM, each chunk's slabs written
+//    while the next one is backprojected, so the volume may exceed HBM;
 //  * one engine context serves every block (the reference creates and destroys
 //    one per block; fdk_initialize reallocates what the new block needs);
 //  * settings attributes that are absent keep their defaults;
@@ -19,6 +21,7 @@
 
 #include <unistd.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -278,9 +281,13 @@ extern "C" int fdk_reconstruct_file(const char *cra_path, const fdk_setting *set
     const unsigned long long slab_bytes = (unsigned long long)rec_x * rec_y * B * sizeof(float);
 
     // angle-block budget: the engine keeps the projections and the filtered
-    // stack in FP32 plus the stored stack (4, 2 or 2 bytes a sample), the whole
-    // volume and one collect slab; 10 % of the free memory is left spare
-    // (para_TC.cpp:65 leaves 10 % of free minus one slab for the projections)
+    // stack in FP32 plus the stored stack (4, 2 or 2 bytes a sample), two
+    // volume chunks (the volume is streamed: a chunk is at most twice
+    // max(Nz/16, 64) + B slices) and one collect slab; 10 % of the free memory
+    // is left spare (para_TC.cpp:65 leaves 10 % of free minus one slab for the
+    // projections)
+    const unsigned long long chunk_bytes =
+        2ULL * (std::max((rec_z + 15) / 16, 64) + B) * rec_x * rec_y * sizeof(float);
     long long budget = opt.angle_block_bytes;
     const size_t free_mem = fdk_get_free_mem_info(set->gpu_id);
     if (budget <= 0) {
@@ -288,15 +295,16 @@ extern "C" int fdk_reconstruct_file(const char *cra_path, const fdk_setting *set
             return fail(FDK_ERR_HIP, "driver: cannot query the free memory of GPU " +
                                          std::to_string(set->gpu_id) + " (no such device?)");
         const double stored = opt.storage == FDK_F32 ? 4.0 : 2.0;
-        const double usable = 0.9 * (double)free_mem - (double)vol_bytes - (double)slab_bytes;
+        const double usable =
+            0.9 * (double)free_mem - 2.0 * (double)chunk_bytes - (double)slab_bytes;
         budget = usable > 0 ? (long long)(usable * 4.0 / (8.0 + stored)) : 0;
     }
     budget = (long long)((unsigned long long)budget / proj_bytes * proj_bytes);
     if (budget < (long long)proj_bytes)
         return fail(FDK_ERR_OUT_OF_MEMORY,
-                    "driver: not even one projection fits beside the volume in free HBM (" +
-                        std::to_string(free_mem >> 20) + " MiB free, volume " +
-                        std::to_string(vol_bytes >> 20) + " MiB)");
+                    "driver: not even one projection fits beside two volume chunks in free HBM (" +
+                        std::to_string(free_mem >> 20) + " MiB free, chunk " +
+                        std::to_string(chunk_bytes >> 20) + " MiB)");
     int angle_blocks = 0;
     if ((rc = fdk_cra_split(cra.p, budget, &angle_blocks))) return rc;
     const int scan_blocks = m.num_of_blocks;
@@ -376,6 +384,7 @@ extern "C" int fdk_reconstruct_file(const char *cra_path, const fdk_setting *set
                                                      0.f, rot)) &&
             !(rc = fdk_set_scale_xyz(c, (float)(1. / m.scale))) &&
             !(rc = fdk_set_storage(c, opt.storage)) &&
+            !(rc = fdk_set_volume_mode(c, FDK_VOLUME_STREAMED)) &&
             !(rc = fdk_set_filter(c, opt.filter, nullptr, 0)) && !(rc = fdk_initialize(c)))
             rc = fdk_calculate_reconstruction(c, fdk_bin_get_block, bin.p);
         st.seconds_engine += seconds_since(te);

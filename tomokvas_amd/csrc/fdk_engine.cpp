@@ -14,6 +14,7 @@
 #include "fdk_internal.hpp"
 
 #include <algorithm>
+#include <numeric>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -39,6 +40,12 @@ struct fdk_ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev[5] = {};
     float stage_ms[4] = {0, 0, 0, 0};
+    // slab pipeline: chunks of the volume leave on copy_stream while the next
+    // chunk is backprojected on stream
+    int volume_mode = FDK_VOLUME_RESIDENT;
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t ev_bp[2] = {}, ev_cp[2] = {};
+    int chunk = 0;           // slices per backprojection chunk (a multiple of B)
 
     // tables (prepared)
     bool prepared = false;
@@ -52,6 +59,7 @@ struct fdk_ctx {
     // engine buffers (initialized)
     bool initialized = false;
     float *d_sino = nullptr, *d_filt = nullptr, *d_vol = nullptr, *d_partial = nullptr;
+    float *d_ring[2] = {nullptr, nullptr};   // streamed volume: two chunk buffers
     void *d_stack = nullptr;
     float *d_scales = nullptr;
     unsigned *d_maxabs = nullptr;
@@ -80,6 +88,7 @@ void free_tables(fdk_ctx *c)
 void free_engine(fdk_ctx *c)
 {
     dfree(c->d_sino); dfree(c->d_filt); dfree(c->d_vol); dfree(c->d_partial);
+    dfree(c->d_ring[0]); dfree(c->d_ring[1]);
     dfree(c->d_stack); dfree(c->d_scales); dfree(c->d_maxabs); dfree(c->d_scratch);
     c->scratch_bytes = 0;
     c->initialized = false;
@@ -244,8 +253,10 @@ int stage_filter(fdk_ctx *c, int dtype, const void *in, void *out, int nang, int
     return tkv::launch_filter(f64 ? 8 : 4, L, st);
 }
 
+// Slices [z0, z0 + nz) of the volume into vol (nz < 0: every slice).
 int stage_backproject(fdk_ctx *c, int store_dtype, const void *stack, const void *scales,
-                      int ang0, int nang, void *vol, int accumulate, hipStream_t st)
+                      int ang0, int nang, void *vol, int accumulate, hipStream_t st,
+                      int z0 = 0, int nz = -1)
 {
     const bool f64 = store_dtype == FDK_F64;
     tkv::BPLaunch L;
@@ -254,7 +265,9 @@ int stage_backproject(fdk_ctx *c, int store_dtype, const void *stack, const void
     L.ang = f64 ? (const void *)(c->d_ang64 + 2 * ang0) : (const void *)(c->d_ang32 + 2 * ang0);
     L.ax = f64 ? (const void *)c->d_ax64 : (const void *)c->d_ax32;
     L.ay = f64 ? (const void *)c->d_ay64 : (const void *)c->d_ay32;
-    L.az = f64 ? (const void *)c->d_az64 : (const void *)c->d_az32;
+    // the z tables run nz_pad >= Nz + 64 entries, so a z-run starting at any
+    // slice stays inside them
+    L.az = f64 ? (const void *)(c->d_az64 + z0) : (const void *)(c->d_az32 + z0);
     L.vol = vol;
     L.Nu = c->g.Nu;
     L.Nv = c->g.Nv;
@@ -262,7 +275,7 @@ int stage_backproject(fdk_ctx *c, int store_dtype, const void *stack, const void
     L.pitch = tkv::stack_pitch(c->g.Nu);
     L.Nx = c->g.Nx;
     L.Ny = c->g.Ny;
-    L.Nz = c->g.Nz;
+    L.Nz = nz < 0 ? c->g.Nz : nz;
     L.cX = c->g.cX;
     L.cZ = c->g.cZ;
     L.D = c->g.D;
@@ -272,10 +285,80 @@ int stage_backproject(fdk_ctx *c, int store_dtype, const void *stack, const void
     return tkv::launch_backproject(store_dtype, L, st);
 }
 
+// Slices per chunk of the slab pipeline: whole slabs of B, about a sixteenth
+// of the volume so that the transfers of the last chunk (the part that cannot
+// overlap a backprojection) stay short, and a multiple of the backprojector's
+// z-run when that costs at most twice the slices.
+int chunk_slices(const tkv::Geometry &g, int zrun)
+{
+    const int target = std::max((g.Nz + 15) / 16, zrun);
+    int ch = (target + g.B - 1) / g.B * g.B;
+    const long long l = std::lcm((long long)g.B, (long long)zrun);
+    const long long cl = (target + l - 1) / l * l;
+    if (cl <= 2LL * ch) ch = (int)cl;
+    return std::min(ch, tkv::slab_count(g.Nz, g.B) * g.B);
+}
+
 int check_angles(fdk_ctx *c, int ang0, int nang)
 {
     if (ang0 < 0 || nang < 0 || ang0 + nang > c->g.Nang)
         return fail(FDK_ERR_INVALID_ARGUMENT, "angle range outside [0, Nang)");
+    return FDK_OK;
+}
+
+// The volume in chunks of c->chunk slices: chunk k+1 is backprojected on the
+// compute stream while the slabs of chunk k go to the host on the copy stream
+// (with a collecting run, each slab's partial sum comes up first and is added
+// on the device) -- the slab loop of kernel.cu:838-884 overlapped with the
+// backprojection.  A streamed volume keeps two chunk buffers in HBM; before
+// chunk k+2 reuses a buffer the compute stream waits for chunk k's transfers.
+// Records ev[3] after the last backprojection and ev[4] after the last
+// transfer, so ev[3] -> ev[4] is the transfer time left exposed.
+int backproject_and_hand_out(fdk_ctx *c, fdk_get_block_fn get_block, void *user)
+{
+    const tkv::Geometry &g = c->g;
+    hipStream_t st = c->stream, cs = c->copy_stream;
+    const bool streamed = c->volume_mode == FDK_VOLUME_STREAMED;
+    const size_t plane = (size_t)g.Nx * g.Ny;
+    const int C = c->chunk, nchunk = (g.Nz + C - 1) / C;
+    const int nslab = tkv::slab_count(g.Nz, g.B);
+    auto buf = [&](int k) { return streamed ? c->d_ring[k & 1] : c->d_vol + (size_t)k * C * plane; };
+    auto bp = [&](int k) -> int {
+        const int z0 = k * C, nz = std::min(C, g.Nz - z0);
+        if (streamed && k >= 2) TKV_HIP(hipStreamWaitEvent(st, c->ev_cp[k & 1], 0));
+        int rc = stage_backproject(c, c->storage, c->d_stack, c->d_scales, 0, g.Nang, buf(k), 0,
+                                   st, z0, nz);
+        if (rc) return rc;
+        TKV_HIP(hipEventRecord(c->ev_bp[k & 1], st));
+        if (k == nchunk - 1) TKV_HIP(hipEventRecord(c->ev[3], st));
+        return FDK_OK;
+    };
+    int rc = bp(0);
+    if (rc) return rc;
+    for (int k = 0; k < nchunk; ++k) {
+        if (k + 1 < nchunk && (rc = bp(k + 1))) return rc;
+        TKV_HIP(hipStreamWaitEvent(cs, c->ev_bp[k & 1], 0));
+        const int n_end = std::min(nslab, (k + 1) * C / g.B);
+        for (int n = k * C / g.B; n < n_end; ++n) {
+            const int nz = std::min(g.B, g.Nz - n * g.B);
+            // the previous slab has landed before its buffer goes to the writer
+            TKV_HIP(hipStreamSynchronize(cs));
+            float *block = get_block(user, n, c->collecting);
+            if (!block)
+                return fail(FDK_ERR_CALLBACK, "slab callback returned NULL for slab " +
+                                                  std::to_string(n));
+            float *dslab = buf(k) + (size_t)(n * g.B - k * C) * plane;
+            if (c->collecting) {
+                TKV_HIP(hipMemcpyAsync(c->d_partial, block, nz * plane * sizeof(float),
+                                       hipMemcpyHostToDevice, cs));
+                if ((rc = tkv::launch_accumulate(dslab, c->d_partial, nz * plane, cs))) return rc;
+            }
+            TKV_HIP(hipMemcpyAsync(block, dslab, nz * plane * sizeof(float),
+                                   hipMemcpyDeviceToHost, cs));
+        }
+        TKV_HIP(hipEventRecord(c->ev_cp[k & 1], cs));
+    }
+    TKV_HIP(hipEventRecord(c->ev[4], cs));
     return FDK_OK;
 }
 
@@ -300,9 +383,14 @@ int fdk_create(int gpu_id, fdk_ctx **out)
     c->dev = gpu_id;
     hipError_t e = hipSetDevice(gpu_id);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking);
     for (int i = 0; i < 5 && e == hipSuccess; ++i) e = hipEventCreate(&c->ev[i]);
+    for (int i = 0; i < 2 && e == hipSuccess; ++i) {
+        e = hipEventCreateWithFlags(&c->ev_bp[i], hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_cp[i], hipEventDisableTiming);
+    }
     if (e != hipSuccess) {
-        delete c;
+        fdk_destroy(c);
         return tkv::hip_fail(e, "fdk_create");
     }
     *out = c;
@@ -314,11 +402,17 @@ int fdk_destroy(fdk_ctx *c)
     if (!c) return FDK_OK;
     (void)hipSetDevice(c->dev);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->copy_stream) (void)hipStreamSynchronize(c->copy_stream);
     free_engine(c);
     free_tables(c);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
+    for (int i = 0; i < 2; ++i) {
+        if (c->ev_bp[i]) (void)hipEventDestroy(c->ev_bp[i]);
+        if (c->ev_cp[i]) (void)hipEventDestroy(c->ev_cp[i]);
+    }
     if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
     delete c;
     return FDK_OK;
 }
@@ -495,14 +589,23 @@ int fdk_initialize(fdk_ctx *c)
     if ((rc = dalloc((void **)&c->d_sino, nsino * sizeof(float), "projection stack")) ||
         (rc = dalloc((void **)&c->d_filt, nsino * sizeof(float), "filtered stack")) ||
         (rc = dalloc(&c->d_stack, fdk_stack_bytes(c, c->storage, g.Nang), "stored stack")) ||
-        (rc = dalloc((void **)&c->d_vol, nvol * sizeof(float), "volume")) ||
         (rc = dalloc((void **)&c->d_scales, g.Nang * sizeof(float), "scales")) ||
         (rc = dalloc((void **)&c->d_maxabs, g.Nang * sizeof(unsigned), "maxabs"))) {
         free_engine(c);
         return rc;
     }
-    if (c->collecting &&
-        (rc = dalloc((void **)&c->d_partial, slab * sizeof(float), "collect slab"))) {
+    c->chunk = chunk_slices(g, tkv::bp_zrun(c->storage));
+    if (c->volume_mode == FDK_VOLUME_STREAMED) {
+        const size_t nchunk = (size_t)c->chunk * g.Nx * g.Ny;
+        const int nring = g.Nz > c->chunk ? 2 : 1;
+        for (int i = 0; i < nring && !rc; ++i)
+            rc = dalloc((void **)&c->d_ring[i], nchunk * sizeof(float), "volume chunk");
+    } else {
+        rc = dalloc((void **)&c->d_vol, nvol * sizeof(float), "volume");
+    }
+    if (!rc && c->collecting)
+        rc = dalloc((void **)&c->d_partial, slab * sizeof(float), "collect slab");
+    if (rc) {
         free_engine(c);
         return rc;
     }
@@ -529,6 +632,24 @@ int fdk_initialize(fdk_ctx *c)
 float *fdk_device_sinogram(fdk_ctx *c) { return c && c->initialized ? c->d_sino : nullptr; }
 const float *fdk_device_volume(fdk_ctx *c) { return c && c->initialized ? c->d_vol : nullptr; }
 
+int fdk_set_volume_mode(fdk_ctx *c, int mode)
+{
+    if (!c) return fail(FDK_ERR_INVALID_ARGUMENT, "null context");
+    if (mode != FDK_VOLUME_RESIDENT && mode != FDK_VOLUME_STREAMED)
+        return fail(FDK_ERR_INVALID_ARGUMENT, "volume mode must be FDK_VOLUME_RESIDENT or "
+                                              "FDK_VOLUME_STREAMED");
+    c->volume_mode = mode;
+    return FDK_OK;
+}
+
+int fdk_get_volume_chunk(fdk_ctx *c, int *slices)
+{
+    if (!c || !slices) return fail(FDK_ERR_INVALID_ARGUMENT, "null argument");
+    if (!c->initialized) return fail(FDK_ERR_NOT_INITIALIZED, "call fdk_initialize first");
+    *slices = c->chunk;
+    return FDK_OK;
+}
+
 int fdk_calculate_reconstruction(fdk_ctx *c, fdk_get_block_fn get_block, void *user)
 {
     if (!c) return fail(FDK_ERR_INVALID_ARGUMENT, "null context");
@@ -547,33 +668,20 @@ int fdk_calculate_reconstruction(fdk_ctx *c, fdk_get_block_fn get_block, void *u
                                c->d_maxabs, c->storage, c->d_stack, c->d_scales, st)))
         return rc;
     TKV_HIP(hipEventRecord(c->ev[2], st));
-    if ((rc = stage_backproject(c, c->storage, c->d_stack, c->d_scales, 0, g.Nang, c->d_vol, 0,
-                                st)))
+    if (!get_block) {
+        if (c->volume_mode == FDK_VOLUME_STREAMED)
+            return fail(FDK_ERR_INVALID_ARGUMENT, "a streamed volume needs a slab callback");
+        if ((rc = stage_backproject(c, c->storage, c->d_stack, c->d_scales, 0, g.Nang, c->d_vol,
+                                    0, st)))
+            return rc;
+        TKV_HIP(hipEventRecord(c->ev[3], st));
+        TKV_HIP(hipEventRecord(c->ev[4], st));
+    } else if ((rc = backproject_and_hand_out(c, get_block, user))) {
+        (void)hipStreamSynchronize(c->copy_stream);
         return rc;
-    TKV_HIP(hipEventRecord(c->ev[3], st));
-    // hand the volume out slab by slab (kernel.cu:838-884)
-    if (get_block) {
-        const int nslab = tkv::slab_count(g.Nz, g.B);
-        const size_t plane = (size_t)g.Nx * g.Ny;
-        for (int n = 0; n < nslab; ++n) {
-            const int nz = std::min(g.B, g.Nz - n * g.B);
-            TKV_HIP(hipStreamSynchronize(st));
-            float *block = get_block(user, n, c->collecting);
-            if (!block)
-                return fail(FDK_ERR_CALLBACK, "slab callback returned NULL for slab " +
-                                                  std::to_string(n));
-            float *dslab = c->d_vol + (size_t)n * g.B * plane;
-            if (c->collecting) {
-                TKV_HIP(hipMemcpyAsync(c->d_partial, block, nz * plane * sizeof(float),
-                                       hipMemcpyHostToDevice, st));
-                if ((rc = tkv::launch_accumulate(dslab, c->d_partial, nz * plane, st))) return rc;
-            }
-            TKV_HIP(hipMemcpyAsync(block, dslab, nz * plane * sizeof(float),
-                                   hipMemcpyDeviceToHost, st));
-        }
     }
-    TKV_HIP(hipEventRecord(c->ev[4], st));
     TKV_HIP(hipStreamSynchronize(st));
+    TKV_HIP(hipStreamSynchronize(c->copy_stream));
     for (int i = 0; i < 4; ++i)
         TKV_HIP(hipEventElapsedTime(&c->stage_ms[i], c->ev[i], c->ev[i + 1]));
     return FDK_OK;

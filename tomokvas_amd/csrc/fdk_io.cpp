@@ -14,6 +14,7 @@
 #include "tomokvas/fdk.h"
 #include "fdk_internal.hpp"
 
+#include <hip/hip_runtime.h>
 #include <sys/types.h>
 #include <unistd.h>
 
@@ -283,6 +284,42 @@ struct IoJob {
     unsigned long long read_pos = 0, read_bytes = 0;
 };
 
+// Host slab buffer: page-locked when the HIP runtime grants it, so that the
+// engine's device-to-host copies into it run at full link speed; ordinary
+// memory otherwise (no GPU present).
+class HostBuf {
+  public:
+    HostBuf() = default;
+    HostBuf(const HostBuf &) = delete;
+    HostBuf &operator=(const HostBuf &) = delete;
+    ~HostBuf() { release(); }
+    bool alloc(size_t n)
+    {
+        release();
+        if (hipHostMalloc((void **)&p_, n * sizeof(float), hipHostMallocDefault) == hipSuccess) {
+            pinned_ = true;
+        } else {
+            (void)hipGetLastError();
+            p_ = (float *)std::malloc(n * sizeof(float));
+            pinned_ = false;
+        }
+        if (p_) std::memset(p_, 0, n * sizeof(float));
+        return p_ != nullptr;
+    }
+    float *data() { return p_; }
+    bool pinned() const { return pinned_; }
+
+  private:
+    void release()
+    {
+        if (p_ && pinned_) (void)hipHostFree(p_);
+        else std::free(p_);
+        p_ = nullptr;
+    }
+    float *p_ = nullptr;
+    bool pinned_ = false;
+};
+
 }  // namespace
 
 struct fdk_bin {
@@ -292,7 +329,7 @@ struct fdk_bin {
     int blocks = 0, rest = 0;
     unsigned long long scan_off = 0;
     FILE *f = nullptr;
-    std::vector<float> buf[3];
+    HostBuf buf[3];
     int work = -1, work_block = -1;   // buffer handed out last, its slab
     int pre = -1, pre_block = -1;     // buffer holding a prefetched slab
     std::thread io;
@@ -371,12 +408,12 @@ extern "C" int fdk_bin_open(const char *dir, const char *name, int x, int y, int
         delete b;
         return fail(FDK_ERR_INVALID_ARGUMENT, "bin: cannot create the volume file: " + why);
     }
-    try {
-        for (auto &v : b->buf) v.assign(b->block_floats, 0.f);
-    } catch (const std::bad_alloc &) {
-        std::fclose(b->f);
-        delete b;
-        return fail(FDK_ERR_OUT_OF_MEMORY, "bin: slab buffers");
+    for (auto &v : b->buf) {
+        if (!v.alloc(b->block_floats)) {
+            std::fclose(b->f);
+            delete b;
+            return fail(FDK_ERR_OUT_OF_MEMORY, "bin: slab buffers");
+        }
     }
     *out = b;
     return FDK_OK;

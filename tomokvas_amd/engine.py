@@ -149,6 +149,16 @@ class FilteredBackProjection:
     def SetStorage(self, dtype: int = F32) -> int:
         return self._L.fdk_set_storage(self._h, dtype)
 
+    def SetVolumeMode(self, mode: int = _lib.VOLUME_RESIDENT) -> int:
+        """VOLUME_RESIDENT (whole volume in HBM) or VOLUME_STREAMED (two chunks
+        in HBM, slabs leave through the sink while the next chunk computes)."""
+        return self._L.fdk_set_volume_mode(self._h, mode)
+
+    def GetVolumeChunk(self) -> int:
+        n = C.c_int()
+        _lib.check(self._L.fdk_get_volume_chunk(self._h, C.byref(n)), "fdk_get_volume_chunk")
+        return n.value
+
     # -- run ----------------------------------------------------------------------------
     def Initialize(self) -> int:
         rc = self._L.fdk_initialize(self._h)
@@ -157,6 +167,9 @@ class FilteredBackProjection:
 
     def CalculateReconstruction(self) -> int:
         sink = self._sink
+        if sink is None:
+            # no sink: the volume stays in HBM (device_volume_ptr)
+            return self._L.fdk_calculate_reconstruction(self._h, _lib.GET_BLOCK_FN(), None)
         native = getattr(sink, "native_get_block", None)
         if native is not None:
             # a C writer (fileio.BinSiz): the engine calls it directly
@@ -165,8 +178,6 @@ class FilteredBackProjection:
         keep = {}
 
         def get_block(user, n, collecting):
-            if sink is None:
-                return None
             buf = sink.GetBlock(n, bool(collecting))
             arr = np.ascontiguousarray(buf, dtype=np.float32)
             if arr is not buf:        # the engine must write into the sink's own memory
