@@ -94,13 +94,17 @@ int fdk_set_scale_xyz(fdk_ctx *ctx, float scale);                  /* SetScaleXY
  * storage type of the filtered stack (default FDK_F32) */
 int fdk_set_filter(fdk_ctx *ctx, int kind, const double *taps, int ntaps);
 int fdk_set_storage(fdk_ctx *ctx, int dtype);
-/* extension: volume residency (default FDK_VOLUME_RESIDENT).  With a slab
- * callback both modes backproject the volume in chunks of whole slabs and move
- * chunk k's slabs to the host on a second stream while chunk k+1 is computed;
- * FDK_VOLUME_STREAMED keeps only two chunks in HBM, so the volume may exceed
- * it (kernel.cu:819-888 computes one slab at a time without overlap). */
+/* extension: volume residency (default FDK_VOLUME_RESIDENT), set before
+ * fdk_initialize.  With a slab callback both modes backproject the volume in
+ * chunks (1/2, 1/4, ... of it when resident; 1/8 when streamed, tapering at
+ * the end) and move the slabs that chunk k completed to the host on a second
+ * stream while chunk k+1 is computed; FDK_VOLUME_STREAMED keeps only about a
+ * quarter of the volume plus one slab in HBM (a circular buffer), so the
+ * volume may exceed HBM
+ * (kernel.cu:819-888 computes one slab at a time without overlap). */
 int fdk_set_volume_mode(fdk_ctx *ctx, int mode);
-/* slices per chunk chosen by fdk_initialize (a multiple of the block size) */
+/* largest backprojection chunk of the slab pipeline, slices (a multiple of
+ * the backprojector's z-run for the current storage type) */
 int fdk_get_volume_chunk(fdk_ctx *ctx, int *slices);
 
 /* ---- engine run ---------------------------------------------------------------------- */

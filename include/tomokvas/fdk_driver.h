@@ -49,6 +49,9 @@ typedef struct {
     long long angle_block_bytes;  /* 0: sized to the free HBM; else the budget of one
                                      angle block, rounded down to whole projections */
     int verbose;                  /* print the reference's progress report          */
+    int volume_mode;              /* -1: streamed when the volume exceeds 45 % of the
+                                     free HBM, else resident; or FDK_VOLUME_RESIDENT /
+                                     FDK_VOLUME_STREAMED                            */
 } fdk_run_options;
 
 typedef struct {
@@ -62,7 +65,7 @@ typedef struct {
 
 /* main() of para_TC.cpp: reconstruct <cra_path> (".../name.CRA") into
  * ".../name.bin" + ".siz".  opt and stats may be NULL (defaults: FDK_F32,
- * Ram-Lak, budget from free HBM, verbose). */
+ * Ram-Lak, budget from free HBM, verbose, residency by size). */
 int fdk_reconstruct_file(const char *cra_path, const fdk_setting *set,
                          const fdk_run_options *opt, fdk_run_stats *stats);
 

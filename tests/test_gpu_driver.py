@@ -95,14 +95,20 @@ def read_volume(folder, shape):
     return np.fromfile(os.path.join(folder, "scan.bin"), np.float32).reshape(shape)
 
 
-@pytest.mark.parametrize("clockwise,storage", [(False, "f32"), (True, "f32"), (True, "f16")])
-def test_file_driver_matches_one_pass(tmp_path, clockwise, storage):
+@pytest.mark.parametrize("clockwise,storage,volume", [(False, "f32", "auto"),
+                                                       (True, "f32", "streamed"),
+                                                       (True, "f16", "auto")])
+def test_file_driver_matches_one_pass(tmp_path, clockwise, storage, volume):
     folder = str(tmp_path)
     cra_path = write_scan(folder, clockwise)
     s = default_setting()
-    s.slices_per_block, s.x_direction, s.vertical_center = 16, 1, 1.5
-    say(f"driver clockwise={clockwise} storage={storage}: 2 scan blocks x 3 angle blocks")
-    st = reconstruct_file(cra_path, s, storage=storage, angle_block_bytes=25 * DX * DZ * 4)
+    # 40 slices of 6: a streamed volume keeps 2 * 32 + 6 = 70 >= 40 slices, i.e.
+    # all of them; the wrapping ring is covered by tests/test_gpu_stream.py
+    s.slices_per_block, s.x_direction, s.vertical_center = 6, 1, 1.5
+    say(f"driver clockwise={clockwise} storage={storage} volume={volume}: 2 scan blocks x 3 "
+        "angle blocks")
+    st = reconstruct_file(cra_path, s, storage=storage, angle_block_bytes=25 * DX * DZ * 4,
+                          volume=volume)
     assert (st.scan_blocks, st.angle_blocks) == (SCANS, 3)          # 25 + 25 + 10
     assert st.bytes_read == SCANS * N * DX * DZ * 4
     got = read_volume(folder, (SCANS, 40, 64, 64))

@@ -117,7 +117,7 @@ class Setting(C.Structure):
 class RunOptions(C.Structure):
     """fdk_run_options (include/tomokvas/fdk_driver.h)."""
     _fields_ = [("storage", C.c_int), ("filter", C.c_int), ("angle_block_bytes", C.c_longlong),
-                ("verbose", C.c_int)]
+                ("verbose", C.c_int), ("volume_mode", C.c_int)]
 
 
 class RunStats(C.Structure):

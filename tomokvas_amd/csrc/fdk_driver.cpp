@@ -8,8 +8,9 @@
 //    down to whole projections so that no projection is left out of a block;
 //  * the next angle block is read from disk on a loader thread while the engine
 //    works on the current one (two host buffers);
-//  * the volume is streamed: two chunks in HBM, each chunk's slabs written
-//    while the next one is backprojected, so the volume may exceed HBM;
+//  * each chunk's slabs are written while the next chunk is backprojected; a
+//    volume above half the free HBM is streamed through a circular buffer of
+//    about a quarter of it, so the volume may exceed HBM;
 //  * one engine context serves every block (the reference creates and destroys
 //    one per block; fdk_initialize reallocates what the new block needs);
 //  * settings attributes that are absent keep their defaults;
@@ -227,7 +228,7 @@ extern "C" int fdk_reconstruct_file(const char *cra_path, const fdk_setting *set
 {
     const Clock::time_point t_start = Clock::now();
     if (!cra_path || !set) return fail(FDK_ERR_INVALID_ARGUMENT, "driver: null argument");
-    fdk_run_options opt{FDK_F32, FDK_FILTER_RAMLAK, 0, 1};
+    fdk_run_options opt{FDK_F32, FDK_FILTER_RAMLAK, 0, 1, -1};
     if (opt_in) opt = *opt_in;
     fdk_run_stats st{};
     if (set->slices_per_block < 1)
@@ -280,31 +281,36 @@ extern "C" int fdk_reconstruct_file(const char *cra_path, const fdk_setting *set
     const unsigned long long vol_bytes = (unsigned long long)rec_x * rec_y * rec_z * sizeof(float);
     const unsigned long long slab_bytes = (unsigned long long)rec_x * rec_y * B * sizeof(float);
 
-    // angle-block budget: the engine keeps the projections and the filtered
-    // stack in FP32 plus the stored stack (4, 2 or 2 bytes a sample), two
-    // volume chunks (the volume is streamed: a chunk is at most twice
-    // max(Nz/16, 64) + B slices) and one collect slab; 10 % of the free memory
-    // is left spare (para_TC.cpp:65 leaves 10 % of free minus one slab for the
-    // projections)
-    const unsigned long long chunk_bytes =
-        2ULL * (std::max((rec_z + 15) / 16, 64) + B) * rec_x * rec_y * sizeof(float);
-    long long budget = opt.angle_block_bytes;
+    // volume residency: whole in HBM when it takes at most half of the free
+    // memory, else streamed through a circular buffer of at most
+    // 2 (Nz/8 + 32) + B slices
     const size_t free_mem = fdk_get_free_mem_info(set->gpu_id);
+    const bool streamed = opt.volume_mode == FDK_VOLUME_STREAMED ||
+                          (opt.volume_mode < 0 && (double)vol_bytes > 0.45 * (double)free_mem);
+    const unsigned long long ring_bytes =
+        (unsigned long long)std::min(2 * ((rec_z + 7) / 8 + 32) + B, rec_z) * rec_x * rec_y *
+        sizeof(float);
+    const unsigned long long held_vol = streamed ? ring_bytes : vol_bytes;
+    // angle-block budget: the engine keeps the projections and the filtered
+    // stack in FP32 plus the stored stack (4, 2 or 2 bytes a sample), the
+    // volume (or its ring) and one collect slab; 10 % of the free memory is
+    // left spare (para_TC.cpp:65 leaves 10 % of free minus one slab for the
+    // projections)
+    long long budget = opt.angle_block_bytes;
     if (budget <= 0) {
         if (free_mem == 0)
             return fail(FDK_ERR_HIP, "driver: cannot query the free memory of GPU " +
                                          std::to_string(set->gpu_id) + " (no such device?)");
         const double stored = opt.storage == FDK_F32 ? 4.0 : 2.0;
-        const double usable =
-            0.9 * (double)free_mem - 2.0 * (double)chunk_bytes - (double)slab_bytes;
+        const double usable = 0.9 * (double)free_mem - (double)held_vol - (double)slab_bytes;
         budget = usable > 0 ? (long long)(usable * 4.0 / (8.0 + stored)) : 0;
     }
     budget = (long long)((unsigned long long)budget / proj_bytes * proj_bytes);
     if (budget < (long long)proj_bytes)
         return fail(FDK_ERR_OUT_OF_MEMORY,
-                    "driver: not even one projection fits beside two volume chunks in free HBM (" +
-                        std::to_string(free_mem >> 20) + " MiB free, chunk " +
-                        std::to_string(chunk_bytes >> 20) + " MiB)");
+                    "driver: not even one projection fits beside the volume in free HBM (" +
+                        std::to_string(free_mem >> 20) + " MiB free, volume held " +
+                        std::to_string(held_vol >> 20) + " MiB)");
     int angle_blocks = 0;
     if ((rc = fdk_cra_split(cra.p, budget, &angle_blocks))) return rc;
     const int scan_blocks = m.num_of_blocks;
@@ -384,7 +390,7 @@ extern "C" int fdk_reconstruct_file(const char *cra_path, const fdk_setting *set
                                                      0.f, rot)) &&
             !(rc = fdk_set_scale_xyz(c, (float)(1. / m.scale))) &&
             !(rc = fdk_set_storage(c, opt.storage)) &&
-            !(rc = fdk_set_volume_mode(c, FDK_VOLUME_STREAMED)) &&
+            !(rc = fdk_set_volume_mode(c, streamed ? FDK_VOLUME_STREAMED : FDK_VOLUME_RESIDENT)) &&
             !(rc = fdk_set_filter(c, opt.filter, nullptr, 0)) && !(rc = fdk_initialize(c)))
             rc = fdk_calculate_reconstruction(c, fdk_bin_get_block, bin.p);
         st.seconds_engine += seconds_since(te);

@@ -40,12 +40,12 @@ struct fdk_ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev[5] = {};
     float stage_ms[4] = {0, 0, 0, 0};
-    // slab pipeline: chunks of the volume leave on copy_stream while the next
-    // chunk is backprojected on stream
+    // slab pipeline: slabs leave on copy_stream while the next chunk of the
+    // volume is backprojected on stream
     int volume_mode = FDK_VOLUME_RESIDENT;
     hipStream_t copy_stream = nullptr;
     hipEvent_t ev_bp[2] = {}, ev_cp[2] = {};
-    int chunk = 0;           // slices per backprojection chunk (a multiple of B)
+    int ring_slices = 0;     // streamed volume: slices held in HBM (circular)
 
     // tables (prepared)
     bool prepared = false;
@@ -59,7 +59,7 @@ struct fdk_ctx {
     // engine buffers (initialized)
     bool initialized = false;
     float *d_sino = nullptr, *d_filt = nullptr, *d_vol = nullptr, *d_partial = nullptr;
-    float *d_ring[2] = {nullptr, nullptr};   // streamed volume: two chunk buffers
+    float *d_ring = nullptr;   // streamed volume: ring_slices slices, slice z at z % ring_slices
     void *d_stack = nullptr;
     float *d_scales = nullptr;
     unsigned *d_maxabs = nullptr;
@@ -88,7 +88,7 @@ void free_tables(fdk_ctx *c)
 void free_engine(fdk_ctx *c)
 {
     dfree(c->d_sino); dfree(c->d_filt); dfree(c->d_vol); dfree(c->d_partial);
-    dfree(c->d_ring[0]); dfree(c->d_ring[1]);
+    dfree(c->d_ring);
     dfree(c->d_stack); dfree(c->d_scales); dfree(c->d_maxabs); dfree(c->d_scratch);
     c->scratch_bytes = 0;
     c->initialized = false;
@@ -285,18 +285,42 @@ int stage_backproject(fdk_ctx *c, int store_dtype, const void *stack, const void
     return tkv::launch_backproject(store_dtype, L, st);
 }
 
-// Slices per chunk of the slab pipeline: whole slabs of B, about a sixteenth
-// of the volume so that the transfers of the last chunk (the part that cannot
-// overlap a backprojection) stay short, and a multiple of the backprojector's
-// z-run when that costs at most twice the slices.
-int chunk_slices(const tkv::Geometry &g, int zrun)
+// Largest backprojection chunk of the slab pipeline: 1/parts of the volume
+// rounded up to a multiple of the backprojector's z-run, so that no chunk
+// ends in a partly filled run.
+int chunk_target(const tkv::Geometry &g, int zrun, int parts)
 {
-    const int target = std::max((g.Nz + 15) / 16, zrun);
-    int ch = (target + g.B - 1) / g.B * g.B;
-    const long long l = std::lcm((long long)g.B, (long long)zrun);
-    const long long cl = (target + l - 1) / l * l;
-    if (cl <= 2LL * ch) ch = (int)cl;
-    return std::min(ch, tkv::slab_count(g.Nz, g.B) * g.B);
+    const int t = std::max((g.Nz + parts - 1) / parts, zrun);
+    return (t + zrun - 1) / zrun * zrun;
+}
+
+// Slices of the circular volume buffer of a streamed volume: two of the
+// largest chunks of any storage type (z-run <= 32) plus one slab, or the
+// whole volume when that is no larger.
+int ring_slices_for(const tkv::Geometry &g)
+{
+    const long long w = 2LL * chunk_target(g, 32, 8) + g.B;
+    return (int)std::min<long long>(w, g.Nz);
+}
+
+// Chunk boundaries z_0 = 0 < z_1 < ... = Nz: chunks of at most C slices; the
+// last ones taper (half the rest, rounded up to the z-run) so that the slab
+// transfers left after the last backprojection are short.  Each launch ends
+// with a partly idle GPU while its last workgroups drain (about 2 ms at
+// 1024^3), so a resident volume starts with half of it: C = Nz/2 gives
+// chunks of 1/2, 1/4, ... down to one z-run.
+std::vector<int> chunk_bounds(int Nz, int C, int zrun)
+{
+    std::vector<int> zb{0};
+    int z = 0;
+    while (z < Nz) {
+        const int rest = Nz - z;
+        int n = std::min(C, std::max(zrun, ((rest + 1) / 2 + zrun - 1) / zrun * zrun));
+        if (n >= rest) n = rest;
+        z += n;
+        zb.push_back(z);
+    }
+    return zb;
 }
 
 int check_angles(fdk_ctx *c, int ang0, int nang)
@@ -306,55 +330,91 @@ int check_angles(fdk_ctx *c, int ang0, int nang)
     return FDK_OK;
 }
 
-// The volume in chunks of c->chunk slices: chunk k+1 is backprojected on the
-// compute stream while the slabs of chunk k go to the host on the copy stream
-// (with a collecting run, each slab's partial sum comes up first and is added
-// on the device) -- the slab loop of kernel.cu:838-884 overlapped with the
-// backprojection.  A streamed volume keeps two chunk buffers in HBM; before
-// chunk k+2 reuses a buffer the compute stream waits for chunk k's transfers.
-// Records ev[3] after the last backprojection and ev[4] after the last
-// transfer, so ev[3] -> ev[4] is the transfer time left exposed.
+// Largest chunk for the current storage type: half the volume when it is
+// resident; an eighth when it is streamed, within 2 C + B <= ring_slices (see
+// backproject_and_hand_out).
+int pipeline_chunk(const fdk_ctx *c)
+{
+    const tkv::Geometry &g = c->g;
+    const int zr = tkv::bp_zrun(c->storage);
+    if (c->volume_mode != FDK_VOLUME_STREAMED) return chunk_target(g, zr, 2);
+    int C = chunk_target(g, zr, 8);
+    if (c->ring_slices < g.Nz) C = std::min(C, (c->ring_slices - g.B) / 2 / zr * zr);
+    return C;
+}
+
+// The volume in chunks: chunk k+1 is backprojected on the compute stream
+// while the slabs that chunk k completed go to the host on the copy stream
+// (with a collecting run each slab's partial sum comes up first and is added
+// on the device) -- the slab loop of kernel.cu:838-884, overlapped with the
+// backprojection.  Chunks are cut at multiples of the z-run, slabs at
+// multiples of B; a slab leaves once the chunk holding its last slice is done.
+//
+// A streamed volume lives in a circular buffer of W = ring_slices slices
+// (slice z at z % W; chunk launches and slab copies that cross the end are
+// split).  With chunks of at most C slices and W >= 2 C + B, chunk k+1
+// overwrites only slices below z_k - B, all in slabs that ended by z_k and so
+// left in iteration k-1: the compute stream waits for that iteration's copies.
+// Records ev[3] after the last backprojection and ev[4] after the last copy,
+// so ev[3] -> ev[4] is the transfer time left exposed.
 int backproject_and_hand_out(fdk_ctx *c, fdk_get_block_fn get_block, void *user)
 {
     const tkv::Geometry &g = c->g;
     hipStream_t st = c->stream, cs = c->copy_stream;
     const bool streamed = c->volume_mode == FDK_VOLUME_STREAMED;
     const size_t plane = (size_t)g.Nx * g.Ny;
-    const int C = c->chunk, nchunk = (g.Nz + C - 1) / C;
+    const int W = streamed ? c->ring_slices : g.Nz;
+    float *ring = streamed ? c->d_ring : c->d_vol;
+    const bool wraps = W < g.Nz;
+    const int C = pipeline_chunk(c);
+    const int zr = tkv::bp_zrun(c->storage);
+    if (C < 1 || (wraps && 2 * C + g.B > W))
+        return fail(FDK_ERR_INVALID_ARGUMENT, "slab pipeline: volume buffer too small");
+    const std::vector<int> zb = chunk_bounds(g.Nz, C, zr);
+    const int nchunk = (int)zb.size() - 1;
     const int nslab = tkv::slab_count(g.Nz, g.B);
-    auto buf = [&](int k) { return streamed ? c->d_ring[k & 1] : c->d_vol + (size_t)k * C * plane; };
+
     auto bp = [&](int k) -> int {
-        const int z0 = k * C, nz = std::min(C, g.Nz - z0);
-        if (streamed && k >= 2) TKV_HIP(hipStreamWaitEvent(st, c->ev_cp[k & 1], 0));
-        int rc = stage_backproject(c, c->storage, c->d_stack, c->d_scales, 0, g.Nang, buf(k), 0,
-                                   st, z0, nz);
-        if (rc) return rc;
+        if (wraps && k >= 2) TKV_HIP(hipStreamWaitEvent(st, c->ev_cp[(k - 2) & 1], 0));
+        for (int z = zb[k]; z < zb[k + 1];) {
+            const int pos = z % W, n = std::min(zb[k + 1] - z, W - pos);
+            int rc = stage_backproject(c, c->storage, c->d_stack, c->d_scales, 0, g.Nang,
+                                       ring + (size_t)pos * plane, 0, st, z, n);
+            if (rc) return rc;
+            z += n;
+        }
         TKV_HIP(hipEventRecord(c->ev_bp[k & 1], st));
         if (k == nchunk - 1) TKV_HIP(hipEventRecord(c->ev[3], st));
         return FDK_OK;
     };
     int rc = bp(0);
     if (rc) return rc;
+    int n = 0;   // next slab to hand out
     for (int k = 0; k < nchunk; ++k) {
         if (k + 1 < nchunk && (rc = bp(k + 1))) return rc;
         TKV_HIP(hipStreamWaitEvent(cs, c->ev_bp[k & 1], 0));
-        const int n_end = std::min(nslab, (k + 1) * C / g.B);
-        for (int n = k * C / g.B; n < n_end; ++n) {
-            const int nz = std::min(g.B, g.Nz - n * g.B);
+        for (; n < nslab && std::min((n + 1) * g.B, g.Nz) <= zb[k + 1]; ++n) {
+            const int z0 = n * g.B, nz = std::min(g.B, g.Nz - z0);
             // the previous slab has landed before its buffer goes to the writer
             TKV_HIP(hipStreamSynchronize(cs));
             float *block = get_block(user, n, c->collecting);
             if (!block)
                 return fail(FDK_ERR_CALLBACK, "slab callback returned NULL for slab " +
                                                   std::to_string(n));
-            float *dslab = buf(k) + (size_t)(n * g.B - k * C) * plane;
-            if (c->collecting) {
+            if (c->collecting)
                 TKV_HIP(hipMemcpyAsync(c->d_partial, block, nz * plane * sizeof(float),
                                        hipMemcpyHostToDevice, cs));
-                if ((rc = tkv::launch_accumulate(dslab, c->d_partial, nz * plane, cs))) return rc;
+            for (int z = z0; z < z0 + nz;) {   // at most two pieces (ring end)
+                const int pos = z % W, m = std::min(z0 + nz - z, W - pos);
+                float *dev = ring + (size_t)pos * plane;
+                const size_t off = (size_t)(z - z0) * plane;
+                if (c->collecting &&
+                    (rc = tkv::launch_accumulate(dev, c->d_partial + off, m * plane, cs)))
+                    return rc;
+                TKV_HIP(hipMemcpyAsync(block + off, dev, m * plane * sizeof(float),
+                                       hipMemcpyDeviceToHost, cs));
+                z += m;
             }
-            TKV_HIP(hipMemcpyAsync(block, dslab, nz * plane * sizeof(float),
-                                   hipMemcpyDeviceToHost, cs));
         }
         TKV_HIP(hipEventRecord(c->ev_cp[k & 1], cs));
     }
@@ -594,12 +654,10 @@ int fdk_initialize(fdk_ctx *c)
         free_engine(c);
         return rc;
     }
-    c->chunk = chunk_slices(g, tkv::bp_zrun(c->storage));
     if (c->volume_mode == FDK_VOLUME_STREAMED) {
-        const size_t nchunk = (size_t)c->chunk * g.Nx * g.Ny;
-        const int nring = g.Nz > c->chunk ? 2 : 1;
-        for (int i = 0; i < nring && !rc; ++i)
-            rc = dalloc((void **)&c->d_ring[i], nchunk * sizeof(float), "volume chunk");
+        c->ring_slices = ring_slices_for(g);
+        rc = dalloc((void **)&c->d_ring, (size_t)c->ring_slices * g.Nx * g.Ny * sizeof(float),
+                    "volume ring");
     } else {
         rc = dalloc((void **)&c->d_vol, nvol * sizeof(float), "volume");
     }
@@ -646,7 +704,7 @@ int fdk_get_volume_chunk(fdk_ctx *c, int *slices)
 {
     if (!c || !slices) return fail(FDK_ERR_INVALID_ARGUMENT, "null argument");
     if (!c->initialized) return fail(FDK_ERR_NOT_INITIALIZED, "call fdk_initialize first");
-    *slices = c->chunk;
+    *slices = pipeline_chunk(c);
     return FDK_OK;
 }
 

@@ -4,6 +4,7 @@
 //   <dir>/<name>.bin and <dir>/<name>.siz.
 // Options beyond the reference: --storage f32|f16|bf16, --shepp-logan,
 // --angle-block-mib N (size of one angle block instead of the free-HBM rule),
+// --streamed / --resident (volume residency instead of the size rule),
 // --setting PATH, --quiet.
 #include "tomokvas/fdk.h"
 #include "tomokvas/fdk_driver.h"
@@ -17,7 +18,7 @@ int main(int argc, char **argv)
 {
     std::printf("MI355X cone beam reconstruction (FDK)\n");
     const char *cra = nullptr, *setting = "setting.xml";
-    fdk_run_options opt{FDK_F32, FDK_FILTER_RAMLAK, 0, 1};
+    fdk_run_options opt{FDK_F32, FDK_FILTER_RAMLAK, 0, 1, -1};
     for (int i = 1; i < argc; ++i) {
         const std::string a = argv[i];
         if (a == "--storage" && i + 1 < argc) {
@@ -35,6 +36,10 @@ int main(int argc, char **argv)
             opt.angle_block_bytes = std::atoll(argv[++i]) * 1024LL * 1024LL;
         } else if (a == "--setting" && i + 1 < argc) {
             setting = argv[++i];
+        } else if (a == "--streamed") {
+            opt.volume_mode = FDK_VOLUME_STREAMED;
+        } else if (a == "--resident") {
+            opt.volume_mode = FDK_VOLUME_RESIDENT;
         } else if (a == "--quiet") {
             opt.verbose = 0;
         } else if (!cra && a.size() && a[0] != '-') {
@@ -48,7 +53,7 @@ int main(int argc, char **argv)
         const char *p = std::strrchr(argv[0], '/');
         std::fprintf(stderr,
                      "usage: %s [--storage f32|f16|bf16] [--shepp-logan] [--angle-block-mib N] "
-                     "[--setting setting.xml] [--quiet] file.cra\n",
+                     "[--streamed|--resident] [--setting setting.xml] [--quiet] file.cra\n",
                      p ? p + 1 : argv[0]);
         return 255;   // main() returns -1
     }

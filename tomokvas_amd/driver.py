@@ -37,14 +37,17 @@ def SaveSetting(setting, path: str = "setting.xml") -> None:
 
 def reconstruct_file(cra_path: str, setting=None, storage: str = "f32",
                      shepp_logan: bool = False, angle_block_bytes: int = 0,
-                     verbose: bool = False) -> "_lib.RunStats":
+                     verbose: bool = False, volume: str = "auto") -> "_lib.RunStats":
     """Reconstruct <dir>/<name>.CRA into <dir>/<name>.bin and .siz; returns the
-    run's statistics (blocks, seconds, bytes)."""
+    run's statistics (blocks, seconds, bytes).  volume: "auto" (streamed when
+    the volume exceeds 45 % of the free HBM), "resident" or "streamed"."""
     if setting is None:
         setting = default_setting()
     opt = _lib.RunOptions(STORAGE[storage],
                           _lib.FILTER_SHEPPLOGAN if shepp_logan else _lib.FILTER_RAMLAK,
-                          int(angle_block_bytes), int(bool(verbose)))
+                          int(angle_block_bytes), int(bool(verbose)),
+                          {"auto": -1, "resident": _lib.VOLUME_RESIDENT,
+                           "streamed": _lib.VOLUME_STREAMED}[volume])
     st = _lib.RunStats()
     _lib.check(_lib.load().fdk_reconstruct_file(cra_path.encode(), C.byref(setting),
                                                  C.byref(opt), C.byref(st)),
