@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libtomokvas_fdk.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "tomokvas", "fdk.h")
 HEADERS = [HEADER] + [os.path.join(os.path.dirname(HERE), "include", "tomokvas", h)
-                      for h in ("fdk_io.h", "fdk_driver.h")]
+                      for h in ("fdk_io.h", "fdk_driver.h", "fdk_rsq.h")]
 
 FDK_OK = 0
 FDK_ERR_INVALID_ARGUMENT = 1
@@ -90,6 +90,14 @@ _SIGS = {
     "fdk_setting_load": (_i, [C.c_char_p, _vp]),
     "fdk_setting_save": (_i, [C.c_char_p, _vp]),
     "fdk_reconstruct_file": (_i, [C.c_char_p, _vp, _vp, _vp]),
+    # Scanco RSQ (fdk_rsq.h)
+    "fdk_rsq_default_params": (_i, [_vp]),
+    "fdk_rsq_open": (_i, [C.c_char_p, C.POINTER(_vp)]),
+    "fdk_rsq_close": (None, [_vp]),
+    "fdk_rsq_get_header": (_i, [_vp, _vp]),
+    "fdk_rsq_sizes": (_i, [_vp, _vp, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i)]),
+    "fdk_rsq_load": (_i, [_vp, _vp, _i, _i, _vp]),
+    "fdk_rsq_process": (_i, [_vp, C.c_double, C.c_double, _vp, _i, _i, _vp, _vp]),
 }
 
 
@@ -126,6 +134,25 @@ class RunStats(C.Structure):
                 ("seconds_total", C.c_double), ("seconds_load_wait", C.c_double),
                 ("seconds_engine", C.c_double), ("seconds_flush", C.c_double),
                 ("bytes_read", C.c_double), ("bytes_written", C.c_double)]
+
+
+class RsqHeader(C.Structure):
+    """fdk_rsq_header (include/tomokvas/fdk_rsq.h)."""
+    _fields_ = [("check", C.c_char * 17)] + [(n, C.c_int) for n in (
+        "data_type", "nr_of_bytes", "nr_of_blocks", "patient_index", "scanner_id", "dimx_p",
+        "dimy_p", "dimz_p", "dimx_um", "dimy_um", "dimz_um", "mu_scaling", "nr_of_samples",
+        "nr_of_projections", "scandist_um", "scanner_type", "reference_line_um", "energy",
+        "intensity", "data_offset")] + [
+        ("bh_x", C.c_int * 3), ("bh_y", C.c_double * 3), ("bh_a", C.c_double),
+        ("bh_b", C.c_double), ("file_size", C.c_longlong)]
+
+
+class RsqParams(C.Structure):
+    """fdk_rsq_params (include/tomokvas/fdk_rsq.h)."""
+    _fields_ = [("sin_x", C.c_int), ("sin_y", C.c_int), ("header_offset", C.c_longlong),
+                ("axis_move", C.c_int), ("distance", C.c_double), ("delta_fi", C.c_double),
+                ("flip", C.c_int), ("invert", C.c_int), ("complete", C.c_int),
+                ("minimal", C.c_int)]
 
 
 _lib = None

@@ -32,6 +32,7 @@ SOURCES = [
     "fdk_probe.hip",
     "fdk_io.cpp",
     "fdk_driver.cpp",
+    "fdk_rsq.hip",
 ]
 # the command line program, linked against the library (host code only)
 MAIN = "fdk_main.cpp"
