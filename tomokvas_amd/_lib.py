@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libtomokvas_fdk.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "tomokvas", "fdk.h")
 HEADERS = [HEADER] + [os.path.join(os.path.dirname(HERE), "include", "tomokvas", h)
-                      for h in ("fdk_io.h", "fdk_driver.h", "fdk_rsq.h")]
+                      for h in ("fdk_io.h", "fdk_driver.h", "fdk_rsq.h", "fdk_tiff.h")]
 
 FDK_OK = 0
 FDK_ERR_INVALID_ARGUMENT = 1
@@ -98,6 +98,12 @@ _SIGS = {
     "fdk_rsq_sizes": (_i, [_vp, _vp, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i)]),
     "fdk_rsq_load": (_i, [_vp, _vp, _i, _i, _vp]),
     "fdk_rsq_process": (_i, [_vp, C.c_double, C.c_double, _vp, _i, _i, _vp, _vp]),
+    # TIFF export (fdk_tiff.h)
+    "fdk_tiff_write_u16": (_i, [C.c_char_p, _vp, _i, _i]),
+    "fdk_volume_minmax": (_i, [_vp, C.c_size_t, C.POINTER(C.c_double), _vp]),
+    "fdk_volume_to_u16": (_i, [_vp, C.c_size_t, C.c_double, C.c_double, _vp, _vp]),
+    "fdk_tiff_export_device": (_i, [_vp, _i, _i, _i, C.c_char_p, _i, C.POINTER(C.c_double)]),
+    "fdk_tiff_export_host": (_i, [_vp, _i, _i, _i, C.c_double, C.c_double, C.c_char_p, _i]),
 }
 
 

@@ -33,6 +33,7 @@ SOURCES = [
     "fdk_io.cpp",
     "fdk_driver.cpp",
     "fdk_rsq.hip",
+    "fdk_tiff.hip",
 ]
 # the command line program, linked against the library (host code only)
 MAIN = "fdk_main.cpp"

@@ -1,0 +1,291 @@
+// TIFF export of reconstructed slices: 16-bit scaling on the GPU, baseline
+// TIFF files written on the host without libtiff.
+//
+// The scaling u = (unsigned short)(65535 * (x - min) / (max - min)) is
+// evaluated in FP64 in that order (no FMA contraction), on the device and on
+// the host alike, so both paths write the same bytes.  Values outside
+// [min, max] clamp to 0 / 65535 and NaN gives 0 (the reference's conversion
+// of such values to unsigned short is undefined).
+#include "tomokvas/fdk.h"
+#include "tomokvas/fdk_tiff.h"
+#include "fdk_internal.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <thread>
+#include <vector>
+
+#pragma clang fp contract(off)
+
+using tkv::fail;
+
+namespace {
+
+__host__ __device__ inline uint16_t scale_u16(float x, double vmin, double vmax)
+{
+    if (vmax == vmin) return 0;
+    const double v = 65535 * ((double)x - vmin) / (vmax - vmin);
+    if (!(v >= 0)) return 0;
+    if (v >= 65535) return 65535;
+    return (uint16_t)(unsigned)v;
+}
+
+// float -> key whose unsigned order is the float order
+__device__ inline unsigned order_key(float x)
+{
+    const unsigned b = __float_as_uint(x);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
+float from_key(unsigned k)
+{
+    const unsigned b = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+    float x;
+    std::memcpy(&x, &b, 4);
+    return x;
+}
+
+__global__ void minmax_init_kernel(unsigned *keys)
+{
+    if (threadIdx.x == 0) {
+        keys[0] = 0xffffffffu;   // running minimum
+        keys[1] = 0u;            // running maximum
+    }
+}
+
+__global__ void __launch_bounds__(256) minmax_kernel(const float *v, size_t n, unsigned *keys)
+{
+    unsigned lo = 0xffffffffu, hi = 0u;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const float x = v[i];
+        if (x == x) {
+            const unsigned k = order_key(x);
+            lo = k < lo ? k : lo;
+            hi = k > hi ? k : hi;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned l2 = __shfl_xor(lo, o), h2 = __shfl_xor(hi, o);
+        lo = l2 < lo ? l2 : lo;
+        hi = h2 > hi ? h2 : hi;
+    }
+    __shared__ unsigned slo[4], shi[4];
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        slo[w] = lo;
+        shi[w] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < 4; ++k) {
+            lo = slo[k] < lo ? slo[k] : lo;
+            hi = shi[k] > hi ? shi[k] : hi;
+        }
+        atomicMin(keys, lo);
+        atomicMax(keys + 1, hi);
+    }
+}
+
+__global__ void __launch_bounds__(256) to_u16_kernel(const float *v, size_t n, double vmin,
+                                                     double vmax, uint16_t *out)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        out[i] = scale_u16(v[i], vmin, vmax);
+}
+
+unsigned grid_for(size_t n) { return (unsigned)std::min<size_t>((n + 255) / 256, 8192); }
+
+void put16(std::vector<unsigned char> &b, size_t at, unsigned v)
+{
+    b[at] = (unsigned char)(v & 0xff);
+    b[at + 1] = (unsigned char)(v >> 8);
+}
+
+void put32(std::vector<unsigned char> &b, size_t at, unsigned v)
+{
+    for (int i = 0; i < 4; ++i) b[at + i] = (unsigned char)(v >> (8 * i));
+}
+
+std::string slice_name(const char *dir, int index)
+{
+    char name[32];
+    std::snprintf(name, sizeof name, "%08d.tif", index);
+    std::string d = dir && *dir ? dir : ".";
+    if (d.back() != '/') d += '/';
+    return d + name;
+}
+
+int write_slices(const uint16_t *img, int x, int y, int count, const char *dir, int first)
+{
+    for (int k = 0; k < count; ++k) {
+        const std::string p = slice_name(dir, first + k);
+        int rc = fdk_tiff_write_u16(p.c_str(), img + (size_t)k * x * y, x, y);
+        if (rc) return rc;
+    }
+    return FDK_OK;
+}
+
+}  // namespace
+
+extern "C" int fdk_tiff_write_u16(const char *path, const uint16_t *img, int width, int height)
+{
+    if (!path || !img || width < 1 || height < 1)
+        return fail(FDK_ERR_INVALID_ARGUMENT, "tiff: null argument or empty image");
+    const unsigned row = 2u * (unsigned)width;
+    if ((unsigned long long)row * height > 0xffff0000ull)
+        return fail(FDK_ERR_UNSUPPORTED, "tiff: image above 4 GiB");
+    // little-endian baseline TIFF: header, one IFD of 10 entries, the strip
+    // offset and byte-count arrays (inline when one row), then the rows
+    const int nent = 10;
+    const size_t ifd = 8, ifd_bytes = 2 + 12 * nent + 4;
+    const bool inline_arrays = height == 1;
+    const size_t offs_at = ifd + ifd_bytes;
+    const size_t counts_at = offs_at + (inline_arrays ? 0 : 4 * (size_t)height);
+    const size_t data_at = counts_at + (inline_arrays ? 0 : 4 * (size_t)height);
+    std::vector<unsigned char> h(data_at, 0);
+    h[0] = 'I';
+    h[1] = 'I';
+    put16(h, 2, 42);
+    put32(h, 4, (unsigned)ifd);
+    put16(h, ifd, nent);
+    size_t e = ifd + 2;
+    auto entry = [&](unsigned tag, unsigned type, unsigned count, unsigned value) {
+        put16(h, e, tag);
+        put16(h, e + 2, type);
+        put32(h, e + 4, count);
+        if (type == 3 && count == 1) put16(h, e + 8, value);
+        else put32(h, e + 8, value);
+        e += 12;
+    };
+    const unsigned SHORT = 3, LONG = 4;
+    entry(256, LONG, 1, (unsigned)width);                       // ImageWidth
+    entry(257, LONG, 1, (unsigned)height);                      // ImageLength
+    entry(258, SHORT, 1, 16);                                   // BitsPerSample
+    entry(259, SHORT, 1, 1);                                    // Compression: none
+    entry(262, SHORT, 1, 1);                                    // Photometric: min-is-black
+    entry(273, LONG, (unsigned)height,                          // StripOffsets
+          inline_arrays ? (unsigned)data_at : (unsigned)offs_at);
+    entry(277, SHORT, 1, 1);                                    // SamplesPerPixel
+    entry(278, LONG, 1, 1);                                     // RowsPerStrip
+    entry(279, LONG, (unsigned)height, inline_arrays ? row : (unsigned)counts_at);  // StripByteCounts
+    entry(284, SHORT, 1, 1);                                    // PlanarConfiguration
+    put32(h, e, 0);                                             // no next IFD
+    if (!inline_arrays) {
+        for (int r = 0; r < height; ++r) {
+            put32(h, offs_at + 4 * (size_t)r, (unsigned)(data_at + (size_t)r * row));
+            put32(h, counts_at + 4 * (size_t)r, row);
+        }
+    }
+    FILE *f = std::fopen(path, "wb");
+    if (!f) return fail(FDK_ERR_INVALID_ARGUMENT, std::string("tiff: cannot write ") + path);
+    bool ok = std::fwrite(h.data(), 1, h.size(), f) == h.size();
+    // samples are little-endian on this host, as the header says
+    ok = ok && std::fwrite(img, 2, (size_t)width * height, f) == (size_t)width * height;
+    ok = std::fclose(f) == 0 && ok;
+    return ok ? FDK_OK : fail(FDK_ERR_INVALID_ARGUMENT, std::string("tiff: short write to ") + path);
+}
+
+extern "C" int fdk_volume_minmax(const float *d_vol, size_t n, double *minmax, void *stream)
+{
+    if (!d_vol || !minmax || n == 0) return fail(FDK_ERR_INVALID_ARGUMENT, "minmax: empty input");
+    hipStream_t st = (hipStream_t)stream;
+    unsigned *keys = nullptr;
+    TKV_HIP(hipMalloc((void **)&keys, 2 * sizeof(unsigned)));
+    unsigned hk[2] = {0, 0};
+    hipLaunchKernelGGL(minmax_init_kernel, dim3(1), dim3(64), 0, st, keys);
+    hipLaunchKernelGGL(minmax_kernel, dim3(grid_for(n)), dim3(256), 0, st, d_vol, n, keys);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(hk, keys, sizeof hk, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(keys);
+    if (e != hipSuccess) return tkv::hip_fail(e, "minmax");
+    if (hk[0] == 0xffffffffu && hk[1] == 0u)
+        return fail(FDK_ERR_INVALID_ARGUMENT, "minmax: no value that is not NaN");
+    minmax[0] = from_key(hk[0]);
+    minmax[1] = from_key(hk[1]);
+    return FDK_OK;
+}
+
+extern "C" int fdk_volume_to_u16(const float *d_vol, size_t n, double vmin, double vmax,
+                                 uint16_t *d_out, void *stream)
+{
+    if (!d_vol || !d_out) return fail(FDK_ERR_INVALID_ARGUMENT, "to_u16: null buffer");
+    if (n == 0) return FDK_OK;
+    hipLaunchKernelGGL(to_u16_kernel, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream,
+                       d_vol, n, vmin, vmax, d_out);
+    TKV_HIP(hipGetLastError());
+    return FDK_OK;
+}
+
+extern "C" int fdk_tiff_export_device(const float *d_vol, int x, int y, int z, const char *dir,
+                                      int first_index, double *minmax_out)
+{
+    if (!d_vol || x < 1 || y < 1 || z < 1)
+        return fail(FDK_ERR_INVALID_ARGUMENT, "tiff export: empty volume");
+    const size_t plane = (size_t)x * y;
+    double mm[2];
+    int rc = fdk_volume_minmax(d_vol, plane * z, mm, nullptr);
+    if (rc) return rc;
+    if (minmax_out) {
+        minmax_out[0] = mm[0];
+        minmax_out[1] = mm[1];
+    }
+    // slabs of about 64 MiB of 16-bit slices: slab k is written on a thread
+    // while slab k+1 is converted and copied
+    const int slab = (int)std::max<size_t>(1, std::min<size_t>(z, (64u << 20) / (2 * plane)));
+    uint16_t *d_u16 = nullptr, *h_u16[2] = {nullptr, nullptr};
+    hipError_t e = hipMalloc((void **)&d_u16, (size_t)slab * plane * 2);
+    for (int i = 0; i < 2 && e == hipSuccess; ++i)
+        e = hipHostMalloc((void **)&h_u16[i], (size_t)slab * plane * 2, hipHostMallocDefault);
+    std::thread writer;
+    int wrc = FDK_OK;
+    std::string werr;
+    auto join = [&]() {
+        if (writer.joinable()) writer.join();
+    };
+    for (int z0 = 0; z0 < z && e == hipSuccess && !rc; z0 += slab) {
+        const int nz = std::min(slab, z - z0);
+        uint16_t *h = h_u16[(z0 / slab) & 1];
+        rc = fdk_volume_to_u16(d_vol + (size_t)z0 * plane, (size_t)nz * plane, mm[0], mm[1], d_u16,
+                               nullptr);
+        if (rc) break;
+        join();   // the buffer written two slabs ago is free again
+        if (wrc) break;
+        e = hipMemcpy(h, d_u16, (size_t)nz * plane * 2, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) break;
+        writer = std::thread([&, h, nz, z0]() {
+            wrc = write_slices(h, x, y, nz, dir, first_index + z0);
+            if (wrc) werr = fdk_last_error_string();
+        });
+    }
+    join();
+    (void)hipFree(d_u16);
+    for (auto p : h_u16)
+        if (p) (void)hipHostFree(p);
+    if (rc) return rc;
+    if (e != hipSuccess) return tkv::hip_fail(e, "tiff export");
+    if (wrc) return fail(wrc, werr);
+    return FDK_OK;
+}
+
+extern "C" int fdk_tiff_export_host(const float *vol, int x, int y, int z, double vmin,
+                                    double vmax, const char *dir, int first_index)
+{
+    if (!vol || x < 1 || y < 1 || z < 1)
+        return fail(FDK_ERR_INVALID_ARGUMENT, "tiff export: empty volume");
+    const size_t plane = (size_t)x * y;
+    std::vector<uint16_t> img(plane);
+    for (int k = 0; k < z; ++k) {
+        const float *s = vol + (size_t)k * plane;
+        for (size_t i = 0; i < plane; ++i) img[i] = scale_u16(s[i], vmin, vmax);
+        int rc = write_slices(img.data(), x, y, 1, dir, first_index + k);
+        if (rc) return rc;
+    }
+    return FDK_OK;
+}
