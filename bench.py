@@ -198,6 +198,9 @@ def main():
     ap.add_argument("--workload", default="cfg3")
     ap.add_argument("--storages", default="f32,f16,bf16")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-slab-pipeline", action="store_true",
+                    help="leave out the run with every slab handed to host memory (its chunked "
+                         "backprojection launches would mix into a kernel trace's averages)")
     ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--calibrate", action="store_true",
                     help="after the timed steps, run accumulate_kernel once over 2^28 floats "
@@ -278,13 +281,16 @@ def main():
         log(f"{NAMES[st]}: filter {filt_ms:.2f} ms, pack {pack_ms:.2f} ms, BP {bp_ms:.2f} ms "
             f"({res[NAMES[st]]['gups']:.1f} GUPS), FDK {fdk_ms:.2f} ms, vs f32 {cmp}")
     hipm.hipFree(ref)
-    pipe_storage = _lib.F16 if "f16" in res else order[0]
-    pipe = slab_pipeline(fbp, g, hipm, pipe_storage, args.steps)
-    r0 = res[NAMES[pipe_storage]]
-    pipe["fdk_without_handout_ms"] = r0["fdk_ms"]
-    pipe["fdk_with_handout_ms"] = pipe["filter_ms"] + pipe["pack_ms"] + pipe["bp_ms"] + \
-        pipe["exposed_transfer_ms"]
-    log(f"slab hand-out ({pipe['storage']}, chunks of {pipe['chunk_slices']} slices): BP "
+    pipe = None
+    if not args.no_slab_pipeline:
+        pipe_storage = _lib.F16 if "f16" in res else order[0]
+        pipe = slab_pipeline(fbp, g, hipm, pipe_storage, args.steps)
+        r0 = res[NAMES[pipe_storage]]
+        pipe["fdk_without_handout_ms"] = r0["fdk_ms"]
+        pipe["fdk_with_handout_ms"] = pipe["filter_ms"] + pipe["pack_ms"] + pipe["bp_ms"] + \
+            pipe["exposed_transfer_ms"]
+    if pipe:
+        log(f"slab hand-out ({pipe['storage']}, chunks of {pipe['chunk_slices']} slices): BP "
         f"{pipe['bp_ms']:.2f} ms, exposed transfers {pipe['exposed_transfer_ms']:.2f} ms, "
         f"wall {pipe['wall_ms']:.1f} ms, slabs equal device volume: "
         f"{pipe['slabs_equal_device_volume']}")
