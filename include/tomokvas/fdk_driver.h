@@ -61,6 +61,7 @@ typedef struct {
     double seconds_engine;        /* initialise + reconstruct, slab hand-over included */
     double seconds_flush;         /* final slab writes and .siz                      */
     double bytes_read, bytes_written;
+    double seconds_initialize;    /* fdk_initialize: allocation and projection upload */
 } fdk_run_stats;
 
 /* main() of para_TC.cpp: reconstruct <cra_path> (".../name.CRA") into

@@ -139,7 +139,8 @@ class RunStats(C.Structure):
     _fields_ = [("scan_blocks", C.c_int), ("angle_blocks", C.c_int),
                 ("seconds_total", C.c_double), ("seconds_load_wait", C.c_double),
                 ("seconds_engine", C.c_double), ("seconds_flush", C.c_double),
-                ("bytes_read", C.c_double), ("bytes_written", C.c_double)]
+                ("bytes_read", C.c_double), ("bytes_written", C.c_double),
+                ("seconds_initialize", C.c_double)]
 
 
 class RsqHeader(C.Structure):

@@ -7,7 +7,8 @@
 //    stored stacks next to the projections, the whole volume resident), rounded
 //    down to whole projections so that no projection is left out of a block;
 //  * the next angle block is read from disk on a loader thread while the engine
-//    works on the current one (two host buffers);
+//    works on the current one (two page-locked host buffers, so the upload runs
+//    at link speed);
 //  * each chunk's slabs are written while the next chunk is backprojected; a
 //    volume above half the free HBM is streamed through a circular buffer of
 //    about a quarter of it, so the volume may exceed HBM;
@@ -111,8 +112,7 @@ struct Block {
     int scan = 0, angle = 0;
     int nang = 0;
     double rot = 0;
-    std::unique_ptr<float[]> data;
-    size_t capacity = 0;             // floats
+    tkv::HostBuffer data;            // page-locked: the engine's upload runs at link speed
     int rc = FDK_OK;
     std::string err;
 };
@@ -128,13 +128,10 @@ void load_block(fdk_cra *cra, size_t proj_floats, Block *b)
     if (!rc) rc = fdk_cra_rotation(cra, &b->rot);
     if (!rc) {
         const size_t need = proj_floats * (size_t)b->nang;
-        if (need > b->capacity) {
-            b->data.reset(new (std::nothrow) float[need]);
-            b->capacity = b->data ? need : 0;
-            if (!b->data) rc = fail(FDK_ERR_OUT_OF_MEMORY, "host buffer for an angle block");
-        }
+        if (need > b->data.size() && !b->data.alloc(need, false))   // overwritten by the load
+            rc = fail(FDK_ERR_OUT_OF_MEMORY, "host buffer for an angle block");
     }
-    if (!rc) rc = fdk_cra_load(cra, b->data.get());
+    if (!rc) rc = fdk_cra_load(cra, b->data.data());
     if (rc) {
         b->rc = rc;
         b->err = fdk_last_error_string();
@@ -376,12 +373,13 @@ extern "C" int fdk_reconstruct_file(const char *cra_path, const fdk_setting *set
                         angle_blocks, rot);
         // para_TC.cpp:134-155
         const Clock::time_point te = Clock::now();
+        Clock::time_point ti = te;
         fdk_ctx *c = ctx.p;
         if (!(rc = fdk_set_block_size(c, B)) &&
             !(rc = fdk_set_allocation_limit(c, (long long)set->alloc_limit_mib * 1024 * 1024)) &&
             !(rc = fdk_set_collecting(c, k > 0)) && !(rc = fdk_set_show_info(c, verbose && i == 0)) &&
             !(rc = fdk_set_axis_direction(c, set->x_direction, set->y_direction, set->z_direction)) &&
-            !(rc = fdk_set_sinogram_properties(c, cur.data.get(), sx, centre, cur.nang, sz,
+            !(rc = fdk_set_sinogram_properties(c, cur.data.data(), sx, centre, cur.nang, sz,
                                                (float)(((double)sz - 1) / 2. + set->vertical_center),
                                                m.delta_fi, m.distance_px,
                                                (float)(m.sample_spacing * 0.0001))) &&
@@ -391,8 +389,11 @@ extern "C" int fdk_reconstruct_file(const char *cra_path, const fdk_setting *set
             !(rc = fdk_set_scale_xyz(c, (float)(1. / m.scale))) &&
             !(rc = fdk_set_storage(c, opt.storage)) &&
             !(rc = fdk_set_volume_mode(c, streamed ? FDK_VOLUME_STREAMED : FDK_VOLUME_RESIDENT)) &&
-            !(rc = fdk_set_filter(c, opt.filter, nullptr, 0)) && !(rc = fdk_initialize(c)))
+            !(rc = fdk_set_filter(c, opt.filter, nullptr, 0)) &&
+            ((ti = Clock::now()), !(rc = fdk_initialize(c)))) {
+            st.seconds_initialize += seconds_since(ti);
             rc = fdk_calculate_reconstruction(c, fdk_bin_get_block, bin.p);
+        }
         st.seconds_engine += seconds_since(te);
         if (rc) return rc;
         if (k == angle_blocks - 1) {

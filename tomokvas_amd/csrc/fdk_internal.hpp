@@ -14,6 +14,8 @@
 #include <hip/hip_runtime.h>
 #include <cstddef>
 #include <cstdint>
+#include <cstdlib>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -135,5 +137,45 @@ struct PhantomLaunch {
 };
 int launch_phantom(const PhantomLaunch &a, hipStream_t st);
 int launch_poisson_log(float *p, size_t n, double I0, uint64_t seed, hipStream_t st);
+
+// Host buffer of floats: page-locked when the HIP runtime grants it, so that
+// copies between it and the device run at full link speed; ordinary memory
+// otherwise (no GPU present).  Contents start at zero unless asked otherwise.
+class HostBuffer {
+  public:
+    HostBuffer() = default;
+    HostBuffer(const HostBuffer &) = delete;
+    HostBuffer &operator=(const HostBuffer &) = delete;
+    ~HostBuffer() { release(); }
+    bool alloc(size_t n, bool zero = true)
+    {
+        release();
+        if (hipHostMalloc((void **)&p_, n * sizeof(float), hipHostMallocDefault) == hipSuccess) {
+            pinned_ = true;
+        } else {
+            (void)hipGetLastError();
+            p_ = (float *)std::malloc(n * sizeof(float));
+            pinned_ = false;
+        }
+        if (p_ && zero) std::memset(p_, 0, n * sizeof(float));
+        n_ = p_ ? n : 0;
+        return p_ != nullptr;
+    }
+    float *data() { return p_; }
+    size_t size() const { return n_; }
+    bool pinned() const { return pinned_; }
+
+  private:
+    void release()
+    {
+        if (p_ && pinned_) (void)hipHostFree(p_);
+        else std::free(p_);
+        p_ = nullptr;
+        n_ = 0;
+    }
+    float *p_ = nullptr;
+    size_t n_ = 0;
+    bool pinned_ = false;
+};
 
 }  // namespace tkv

@@ -15,6 +15,7 @@
 #include "fdk_internal.hpp"
 
 #include <hip/hip_runtime.h>
+#include <fcntl.h>
 #include <sys/types.h>
 #include <unistd.h>
 
@@ -243,8 +244,8 @@ extern "C" int fdk_cra_load(const fdk_cra *cra, float *dst)
     if (!cra || !dst) return fail(FDK_ERR_INVALID_ARGUMENT, "cra: null argument");
     const fdk_cra_meta &m = cra->m;
     const std::string fn = join_path(cra->dir.c_str(), m.raw_file);
-    FILE *f = std::fopen(fn.c_str(), "rb");
-    if (!f) return fail(FDK_ERR_INVALID_ARGUMENT, "cra: cannot open raw file " + fn);
+    const int fd = open(fn.c_str(), O_RDONLY);
+    if (fd < 0) return fail(FDK_ERR_INVALID_ARGUMENT, "cra: cannot open raw file " + fn);
     const size_t proj = (size_t)m.detector_x * m.detector_z;
     const int n = projections_of(cra);
     const size_t sin = (size_t)m.detector_x * n;   // one sinogram row of the block
@@ -254,23 +255,43 @@ extern "C" int fdk_cra_load(const fdk_cra *cra, float *dst)
     const unsigned long long off =
         ((unsigned long long)cra->active * cra->per_block * proj + scan * cra->scan_block) *
         sizeof(float);
-    if (fseeko(f, (off_t)off, SEEK_SET) != 0) {
-        std::fclose(f);
-        return fail(FDK_ERR_INVALID_ARGUMENT, "cra: seek beyond the raw file");
-    }
-    std::vector<float> p(proj);
-    for (int j = 0; j < n; ++j) {
-        if (std::fread(p.data(), sizeof(float), proj, f) != proj) {
-            std::fclose(f);
+    // projections [j0, j1) per thread: positioned reads of whole projections,
+    // each scattered row by row into the sinogram-major block; the reads and
+    // the scatter are memory/disk bound, so several run at once
+    const int nthreads =
+        std::max(1, std::min({n, 16, (int)std::max(1u, std::thread::hardware_concurrency())}));
+    std::vector<int> bad(nthreads, 0);
+    auto work = [&](int t) {
+        std::vector<float> p(proj);
+        const int j0 = (int)((long long)n * t / nthreads), j1 = (int)((long long)n * (t + 1) / nthreads);
+        for (int j = j0; j < j1 && !bad[t]; ++j) {
+            const size_t want = proj * sizeof(float);
+            size_t got = 0;
+            while (got < want) {
+                const ssize_t r = pread(fd, (char *)p.data() + got, want - got,
+                                        (off_t)(off + (unsigned long long)j * want + got));
+                if (r <= 0) break;
+                got += (size_t)r;
+            }
+            if (got != want) {
+                bad[t] = 1;
+                break;
+            }
+            const int a = m.clockwise ? n - 1 - j : j;
+            for (int s = 0; s < m.detector_z; ++s)
+                std::memcpy(dst + sin * s + (size_t)a * m.detector_x,
+                            p.data() + (size_t)s * m.detector_x, m.detector_x * sizeof(float));
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nthreads; ++t) pool.emplace_back(work, t);
+    work(0);
+    for (auto &th : pool) th.join();
+    close(fd);
+    for (int v : bad)
+        if (v)
             return fail(FDK_ERR_INVALID_ARGUMENT,
                         "cra: raw file shorter than its header says (" + fn + ")");
-        }
-        const int a = m.clockwise ? n - 1 - j : j;
-        for (int s = 0; s < m.detector_z; ++s)
-            std::memcpy(dst + sin * s + (size_t)a * m.detector_x, p.data() + (size_t)s * m.detector_x,
-                        m.detector_x * sizeof(float));
-    }
-    std::fclose(f);
     return FDK_OK;
 }
 
@@ -284,42 +305,6 @@ struct IoJob {
     unsigned long long read_pos = 0, read_bytes = 0;
 };
 
-// Host slab buffer: page-locked when the HIP runtime grants it, so that the
-// engine's device-to-host copies into it run at full link speed; ordinary
-// memory otherwise (no GPU present).
-class HostBuf {
-  public:
-    HostBuf() = default;
-    HostBuf(const HostBuf &) = delete;
-    HostBuf &operator=(const HostBuf &) = delete;
-    ~HostBuf() { release(); }
-    bool alloc(size_t n)
-    {
-        release();
-        if (hipHostMalloc((void **)&p_, n * sizeof(float), hipHostMallocDefault) == hipSuccess) {
-            pinned_ = true;
-        } else {
-            (void)hipGetLastError();
-            p_ = (float *)std::malloc(n * sizeof(float));
-            pinned_ = false;
-        }
-        if (p_) std::memset(p_, 0, n * sizeof(float));
-        return p_ != nullptr;
-    }
-    float *data() { return p_; }
-    bool pinned() const { return pinned_; }
-
-  private:
-    void release()
-    {
-        if (p_ && pinned_) (void)hipHostFree(p_);
-        else std::free(p_);
-        p_ = nullptr;
-    }
-    float *p_ = nullptr;
-    bool pinned_ = false;
-};
-
 }  // namespace
 
 struct fdk_bin {
@@ -329,7 +314,7 @@ struct fdk_bin {
     int blocks = 0, rest = 0;
     unsigned long long scan_off = 0;
     FILE *f = nullptr;
-    HostBuf buf[3];
+    tkv::HostBuffer buf[3];
     int work = -1, work_block = -1;   // buffer handed out last, its slab
     int pre = -1, pre_block = -1;     // buffer holding a prefetched slab
     std::thread io;
