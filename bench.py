@@ -55,6 +55,50 @@ def hip():
     return h
 
 
+def end_to_end(fbp, g, hipm, storage, steps):
+    """The projections handed over in host memory (page-locked), as the
+    engine's boundary receives them: fdk_initialize's upload followed by the
+    FDK (the reference's order) against fdk_reconstruct_host, which uploads
+    angle chunks behind the filtering and backprojection.  Both include the
+    engine's allocations; the volume stays in HBM."""
+    L = _lib.load()
+    n = g.Nu * g.Nv * g.Nang
+    host = C.c_void_p()
+    assert hipm.hipHostMalloc(C.byref(host), n * 4, 0) == 0
+    assert hipm.hipMemcpy(host, C.c_void_p(fbp.device_sinogram_ptr()), n * 4, 2) == 0
+    arr = np.ctypeslib.as_array(C.cast(host, C.POINTER(C.c_float)), (n,))
+    _lib.check(fbp.SetStorage(storage), "SetStorage")
+    nullcb = _lib.GET_BLOCK_FN()
+    nvol = g.Nx * g.Ny * g.Nz
+    ref = C.c_void_p()
+    assert hipm.hipMalloc(C.byref(ref), nvol * 4) == 0
+    serial, fused = [], []
+    for _ in range(steps + 1):
+        g.apply(fbp, arr, None)
+        t0 = time.perf_counter()
+        _lib.check(fbp.Initialize(), "Initialize")
+        _lib.check(L.fdk_calculate_reconstruction(fbp.handle, nullcb, None), "calculate")
+        hipm.hipDeviceSynchronize()
+        serial.append(time.perf_counter() - t0)
+    assert hipm.hipMemcpy(ref, C.c_void_p(fbp.device_volume_ptr()), nvol * 4, 3) == 0
+    for _ in range(steps + 1):
+        g.apply(fbp, None, None)
+        t0 = time.perf_counter()
+        _lib.check(L.fdk_reconstruct_host(fbp.handle, host, nullcb, None), "reconstruct_host")
+        hipm.hipDeviceSynchronize()
+        fused.append(time.perf_counter() - t0)
+    out3 = (C.c_double * 3)()
+    _lib.check(L.fdk_compare(C.c_void_p(fbp.device_volume_ptr()), ref, nvol, out3, None),
+               "compare")
+    hipm.hipFree(ref)
+    hipm.hipHostFree(host)
+    ts, tf = float(np.mean(serial[1:])), float(np.mean(fused[1:]))
+    return {"storage": NAMES[storage], "host": "page-locked", "projection_bytes": n * 4,
+            "upload_then_fdk_s": ts, "overlapped_s": tf,
+            "end_to_end_gups": g.voxel_updates / tf / 1e9,
+            "rel_rmse_overlapped_vs_serial": math.sqrt(out3[0] / out3[1])}
+
+
 def slab_pipeline(fbp, g, hipm, storage, steps):
     """FDK with every slab handed to the host (page-locked memory, the writer's
     case): the backprojection runs in chunks while the previous chunk's slabs
@@ -198,6 +242,8 @@ def main():
     ap.add_argument("--workload", default="cfg3")
     ap.add_argument("--storages", default="f32,f16,bf16")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-end-to-end", action="store_true",
+                    help="leave out the runs that start from projections in host memory")
     ap.add_argument("--no-slab-pipeline", action="store_true",
                     help="leave out the run with every slab handed to host memory (its chunked "
                          "backprojection launches would mix into a kernel trace's averages)")
@@ -349,6 +395,14 @@ def main():
         out["cpu_baseline"] = cpu
         out["rel_rmse"].update(errs)
         log(f"cpu leg {time.perf_counter() - t0:.1f} s: {cpu['value']:.3f} GUPS, errors {errs}")
+    if not args.no_end_to_end:
+        e2e_storage = _lib.F16 if "f16" in res else order[0]
+        t0 = time.perf_counter()
+        out["end_to_end"] = e2e = end_to_end(fbp, g, hipm, e2e_storage, args.steps)
+        log(f"end to end from host memory ({e2e['storage']}): upload then FDK "
+            f"{e2e['upload_then_fdk_s']:.3f} s, overlapped {e2e['overlapped_s']:.3f} s "
+            f"({e2e['end_to_end_gups']:.0f} GUPS), rel RMSE {e2e['rel_rmse_overlapped_vs_serial']:.1e}"
+            f"; leg {time.perf_counter() - t0:.1f} s")
     if args.calibrate:
         n = 1 << 28
         dst = fbp.device_sinogram_ptr()

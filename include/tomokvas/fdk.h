@@ -117,6 +117,16 @@ int fdk_initialize(fdk_ctx *ctx);
  * visits ceil((Nz-1)/B) slabs and so skips the last slice when (Nz-1) % B == 0;
  * this engine always covers every slice. */
 int fdk_calculate_reconstruction(fdk_ctx *ctx, fdk_get_block_fn get_block, void *user);
+/* extension: fdk_initialize + fdk_calculate_reconstruction in one call with
+ * the host stack [v][beta][u] borrowed for its duration (set the sinogram
+ * properties with a NULL pointer).  With a resident volume the angles are cut
+ * into up to 8 chunks of >= 100 angles and chunk k+1 is uploaded while chunk k
+ * is filtered and backprojected, so the host-link transfer hides behind the
+ * computation (page-locked host memory transfers fastest).  Stage times:
+ * [0], [1] zero, [2] first upload to last backprojection, [3] slab transfers
+ * left after it.  fdk_device_sinogram's contents are unspecified afterwards. */
+int fdk_reconstruct_host(fdk_ctx *ctx, const float *host_sino, fdk_get_block_fn get_block,
+                         void *user);
 /* Device copy of the last reconstructed volume [Nz][Ny][Nx] (owned by ctx). */
 const float *fdk_device_volume(fdk_ctx *ctx);
 /* Device projection stack [v][beta][u] of an initialised context.  With a NULL

@@ -58,10 +58,10 @@ typedef struct {
     int scan_blocks, angle_blocks;
     double seconds_total;
     double seconds_load_wait;     /* main thread waiting on the projection loader   */
-    double seconds_engine;        /* initialise + reconstruct, slab hand-over included */
+    double seconds_engine;        /* upload + reconstruct (fdk_reconstruct_host), slab
+                                     hand-over included */
     double seconds_flush;         /* final slab writes and .siz                      */
     double bytes_read, bytes_written;
-    double seconds_initialize;    /* fdk_initialize: allocation and projection upload */
 } fdk_run_stats;
 
 /* main() of para_TC.cpp: reconstruct <cra_path> (".../name.CRA") into

@@ -34,7 +34,8 @@ def d2h(ptr, shape):
 
 
 def one_launch(g, s, storage):
-    """No sink: one backprojection over the whole volume, read from HBM."""
+    """No sink: the whole volume backprojected (in angle chunks of >= 100
+    angles, at most 8), read from HBM."""
     fbp = engine(g, s, None, storage=storage)
     _lib.check(fbp.Initialize(), "Initialize")
     _lib.check(fbp.CalculateReconstruction(), "CalculateReconstruction")
@@ -110,3 +111,33 @@ def test_streamed_volume_needs_a_sink():
     assert fbp.CalculateReconstruction() == _lib.FDK_ERR_INVALID_ARGUMENT
     assert fbp.SetVolumeMode(7) == _lib.FDK_ERR_INVALID_ARGUMENT
     fbp.close()
+
+
+@pytest.mark.parametrize("storage", [_lib.F32, _lib.F16])
+@pytest.mark.parametrize("with_sink", [False, True])
+def test_reconstruct_host_overlaps_upload(storage, with_sink):
+    # 360 angles: three chunks of 120, each uploaded while the previous one is
+    # filtered and backprojected; the chunks' partial volumes are summed, as
+    # the one-pass run without a sink also does
+    g = scan(64, 48, 360, 48, 44, 40, B=8)
+    rng = np.random.default_rng(9)
+    s = rng.standard_normal((g.Nv, g.Nang, g.Nu)).astype(np.float32)
+    want = one_launch(g, s, storage)            # three angle chunks as well
+    sink = VolumeSink(g.Nx, g.Ny, g.Nz, g.B) if with_sink else None
+    fbp = engine(g, None, sink, storage=storage)
+    say(f"reconstruct_host storage={storage} sink={with_sink}")
+    _lib.check(fbp.ReconstructHost(s), "ReconstructHost")
+    got = sink.volume if with_sink else d2h(fbp.device_volume_ptr(), (g.Nz, g.Ny, g.Nx))
+    times = fbp.stage_times_ms()
+    fbp.close()
+    e = O.rel_rmse(got, want)
+    say(f"  rel RMSE vs one pass {e:.2e}; upload..BP {times[2]:.3f} ms, exposed {times[3]:.3f} ms")
+    assert e <= 1e-6
+    # a streamed volume takes one chunk (one launch over all angles)
+    if with_sink:
+        sink2 = VolumeSink(g.Nx, g.Ny, g.Nz, g.B)
+        fbp = engine(g, None, sink2, storage=storage)
+        _lib.check(fbp.SetVolumeMode(_lib.VOLUME_STREAMED))
+        _lib.check(fbp.ReconstructHost(s), "ReconstructHost streamed")
+        fbp.close()
+        assert O.rel_rmse(sink2.volume, want) <= 1e-6

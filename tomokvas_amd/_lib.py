@@ -51,6 +51,7 @@ _SIGS = {
     "fdk_set_storage": (_i, [_vp, _i]),
     "fdk_initialize": (_i, [_vp]),
     "fdk_calculate_reconstruction": (_i, [_vp, GET_BLOCK_FN, _vp]),
+    "fdk_reconstruct_host": (_i, [_vp, _vp, GET_BLOCK_FN, _vp]),
     "fdk_device_volume": (_vp, [_vp]),
     "fdk_device_sinogram": (_vp, [_vp]),
     "fdk_get_stage_times": (_i, [_vp, _vp]),
@@ -139,8 +140,7 @@ class RunStats(C.Structure):
     _fields_ = [("scan_blocks", C.c_int), ("angle_blocks", C.c_int),
                 ("seconds_total", C.c_double), ("seconds_load_wait", C.c_double),
                 ("seconds_engine", C.c_double), ("seconds_flush", C.c_double),
-                ("bytes_read", C.c_double), ("bytes_written", C.c_double),
-                ("seconds_initialize", C.c_double)]
+                ("bytes_read", C.c_double), ("bytes_written", C.c_double)]
 
 
 class RsqHeader(C.Structure):

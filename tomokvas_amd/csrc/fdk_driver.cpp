@@ -371,15 +371,16 @@ extern "C" int fdk_reconstruct_file(const char *cra_path, const fdk_setting *set
         if (verbose)
             std::printf("calculating sinogram block %d / %d\n  rotation correction: %g rad\n", k + 1,
                         angle_blocks, rot);
-        // para_TC.cpp:134-155
+        // para_TC.cpp:134-155, with Initialize and CalculateReconstruction
+        // fused so that the upload of the block runs behind its filtering and
+        // backprojection
         const Clock::time_point te = Clock::now();
-        Clock::time_point ti = te;
         fdk_ctx *c = ctx.p;
         if (!(rc = fdk_set_block_size(c, B)) &&
             !(rc = fdk_set_allocation_limit(c, (long long)set->alloc_limit_mib * 1024 * 1024)) &&
             !(rc = fdk_set_collecting(c, k > 0)) && !(rc = fdk_set_show_info(c, verbose && i == 0)) &&
             !(rc = fdk_set_axis_direction(c, set->x_direction, set->y_direction, set->z_direction)) &&
-            !(rc = fdk_set_sinogram_properties(c, cur.data.data(), sx, centre, cur.nang, sz,
+            !(rc = fdk_set_sinogram_properties(c, nullptr, sx, centre, cur.nang, sz,
                                                (float)(((double)sz - 1) / 2. + set->vertical_center),
                                                m.delta_fi, m.distance_px,
                                                (float)(m.sample_spacing * 0.0001))) &&
@@ -389,11 +390,8 @@ extern "C" int fdk_reconstruct_file(const char *cra_path, const fdk_setting *set
             !(rc = fdk_set_scale_xyz(c, (float)(1. / m.scale))) &&
             !(rc = fdk_set_storage(c, opt.storage)) &&
             !(rc = fdk_set_volume_mode(c, streamed ? FDK_VOLUME_STREAMED : FDK_VOLUME_RESIDENT)) &&
-            !(rc = fdk_set_filter(c, opt.filter, nullptr, 0)) &&
-            ((ti = Clock::now()), !(rc = fdk_initialize(c)))) {
-            st.seconds_initialize += seconds_since(ti);
-            rc = fdk_calculate_reconstruction(c, fdk_bin_get_block, bin.p);
-        }
+            !(rc = fdk_set_filter(c, opt.filter, nullptr, 0)))
+            rc = fdk_reconstruct_host(c, cur.data.data(), fdk_bin_get_block, bin.p);
         st.seconds_engine += seconds_since(te);
         if (rc) return rc;
         if (k == angle_blocks - 1) {

@@ -44,7 +44,7 @@ struct fdk_ctx {
     // volume is backprojected on stream
     int volume_mode = FDK_VOLUME_RESIDENT;
     hipStream_t copy_stream = nullptr;
-    hipEvent_t ev_bp[2] = {}, ev_cp[2] = {};
+    hipEvent_t ev_bp[2] = {}, ev_cp[2] = {}, ev_up[2] = {};
     int ring_slices = 0;     // streamed volume: slices held in HBM (circular)
 
     // tables (prepared)
@@ -65,6 +65,11 @@ struct fdk_ctx {
     unsigned *d_maxabs = nullptr;
     void *d_scratch = nullptr;
     size_t scratch_bytes = 0;
+    // bytes held by each engine buffer: fdk_initialize keeps a buffer whose
+    // capacity still fits (freeing and allocating tens of GB on every angle
+    // block has been seen to slow down from milliseconds to seconds per call)
+    size_t cap_sino = 0, cap_filt = 0, cap_stack = 0, cap_scales = 0, cap_maxabs = 0;
+    size_t cap_vol = 0, cap_ring = 0, cap_partial = 0;
 };
 
 namespace {
@@ -91,6 +96,8 @@ void free_engine(fdk_ctx *c)
     dfree(c->d_ring);
     dfree(c->d_stack); dfree(c->d_scales); dfree(c->d_maxabs); dfree(c->d_scratch);
     c->scratch_bytes = 0;
+    c->cap_sino = c->cap_filt = c->cap_stack = c->cap_scales = c->cap_maxabs = 0;
+    c->cap_vol = c->cap_ring = c->cap_partial = 0;
     c->initialized = false;
 }
 
@@ -101,6 +108,21 @@ int upload(T *&d, const std::vector<T> &h)
     TKV_HIP(hipMalloc((void **)&d, sizeof(T) * std::max<size_t>(h.size(), 1)));
     if (!h.empty()) TKV_HIP(hipMemcpy(d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
     return FDK_OK;
+}
+
+int dalloc(void **p, size_t bytes, const char *what);
+
+// *p holds at least `need` bytes afterwards: kept when its capacity fits,
+// else freed and allocated again
+int ensure(void **p, size_t &cap, size_t need, const char *what)
+{
+    if (*p && cap >= need) return FDK_OK;
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    cap = 0;
+    int rc = dalloc(p, need, what);
+    if (!rc) cap = need;
+    return rc;
 }
 
 int dalloc(void **p, size_t bytes, const char *what)
@@ -330,6 +352,42 @@ int check_angles(fdk_ctx *c, int ang0, int nang)
     return FDK_OK;
 }
 
+// Angle chunks of a whole-volume backprojection: the angle loop split into
+// up to TKV_BP_ANGLE_CHUNKS launches of at least TKV_BP_CHUNK_MIN_ANGLES
+// angles, the later ones adding onto the volume.  Within one long launch the
+// workgroups drift apart in their progress through the angles, so the
+// projections read at any moment spread over many angles; a launch boundary
+// every ~200 angles keeps them together (measured at 1024^3 from 1800
+// projections: 500 ms in one launch, 476 ms in eight).
+#ifndef TKV_BP_ANGLE_CHUNKS
+#define TKV_BP_ANGLE_CHUNKS 8
+#endif
+#ifndef TKV_BP_CHUNK_MIN_ANGLES
+#define TKV_BP_CHUNK_MIN_ANGLES 100
+#endif
+std::vector<int> angle_bounds(int nang)
+{
+    const int K = std::max(1, std::min(TKV_BP_ANGLE_CHUNKS, nang / TKV_BP_CHUNK_MIN_ANGLES));
+    std::vector<int> ab(K + 1);
+    for (int k = 0; k <= K; ++k) ab[k] = (int)((long long)nang * k / K);
+    return ab;
+}
+
+// Angles [ab[k], ab[k+1]) of the stored stack onto vol, each chunk after the
+// first adding onto it (the first also when accumulate is set).
+int backproject_in_angle_chunks(fdk_ctx *c, const std::vector<int> &ab, void *vol,
+                                int accumulate, hipStream_t st)
+{
+    const size_t stored = (size_t)sample_bytes(c->storage) * tkv::stack_pitch(c->g.Nu) * c->g.Nv;
+    for (size_t k = 0; k + 1 < ab.size(); ++k) {
+        const int a0 = ab[k], nc = ab[k + 1] - a0;
+        int rc = stage_backproject(c, c->storage, (const char *)c->d_stack + (size_t)a0 * stored,
+                                   c->d_scales + a0, a0, nc, vol, accumulate || k > 0, st);
+        if (rc) return rc;
+    }
+    return FDK_OK;
+}
+
 // Largest chunk for the current storage type: half the volume when it is
 // resident; an eighth when it is streamed, within 2 C + B <= ring_slices (see
 // backproject_and_hand_out).
@@ -357,7 +415,11 @@ int pipeline_chunk(const fdk_ctx *c)
 // left in iteration k-1: the compute stream waits for that iteration's copies.
 // Records ev[3] after the last backprojection and ev[4] after the last copy,
 // so ev[3] -> ev[4] is the transfer time left exposed.
-int backproject_and_hand_out(fdk_ctx *c, fdk_get_block_fn get_block, void *user)
+// Angles [ang0, ang0 + nang) of the stored stack (stack and scales point at
+// angle ang0), added onto the volume when accumulate is set.
+int backproject_and_hand_out(fdk_ctx *c, fdk_get_block_fn get_block, void *user,
+                             const void *stack, const float *scales, int ang0, int nang,
+                             int accumulate)
 {
     const tkv::Geometry &g = c->g;
     hipStream_t st = c->stream, cs = c->copy_stream;
@@ -378,8 +440,8 @@ int backproject_and_hand_out(fdk_ctx *c, fdk_get_block_fn get_block, void *user)
         if (wraps && k >= 2) TKV_HIP(hipStreamWaitEvent(st, c->ev_cp[(k - 2) & 1], 0));
         for (int z = zb[k]; z < zb[k + 1];) {
             const int pos = z % W, n = std::min(zb[k + 1] - z, W - pos);
-            int rc = stage_backproject(c, c->storage, c->d_stack, c->d_scales, 0, g.Nang,
-                                       ring + (size_t)pos * plane, 0, st, z, n);
+            int rc = stage_backproject(c, c->storage, stack, scales, ang0, nang,
+                                       ring + (size_t)pos * plane, accumulate, st, z, n);
             if (rc) return rc;
             z += n;
         }
@@ -448,6 +510,7 @@ int fdk_create(int gpu_id, fdk_ctx **out)
     for (int i = 0; i < 2 && e == hipSuccess; ++i) {
         e = hipEventCreateWithFlags(&c->ev_bp[i], hipEventDisableTiming);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_cp[i], hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_up[i], hipEventDisableTiming);
     }
     if (e != hipSuccess) {
         fdk_destroy(c);
@@ -470,6 +533,7 @@ int fdk_destroy(fdk_ctx *c)
     for (int i = 0; i < 2; ++i) {
         if (c->ev_bp[i]) (void)hipEventDestroy(c->ev_bp[i]);
         if (c->ev_cp[i]) (void)hipEventDestroy(c->ev_cp[i]);
+        if (c->ev_up[i]) (void)hipEventDestroy(c->ev_up[i]);
     }
     if (c->stream) (void)hipStreamDestroy(c->stream);
     if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
@@ -611,9 +675,8 @@ int fdk_set_storage(fdk_ctx *c, int dtype)
     if (dtype != FDK_F32 && dtype != FDK_F16 && dtype != FDK_BF16)
         return fail(FDK_ERR_INVALID_ARGUMENT, "storage must be F32, F16 or BF16");
     if (c->initialized && dtype != c->storage) {
-        dfree(c->d_stack);
-        size_t bytes = fdk_stack_bytes(c, dtype, c->g.Nang);
-        int rc = dalloc(&c->d_stack, bytes, "stored stack");
+        int rc = ensure(&c->d_stack, c->cap_stack, fdk_stack_bytes(c, dtype, c->g.Nang),
+                        "stored stack");
         if (rc) return rc;
     }
     c->storage = dtype;
@@ -641,28 +704,33 @@ int fdk_initialize(fdk_ctx *c)
     if (!c) return fail(FDK_ERR_INVALID_ARGUMENT, "null context");
     int rc = prepare(c);
     if (rc) return rc;
-    free_engine(c);
+    c->initialized = false;
     const tkv::Geometry &g = c->g;
     const size_t nsino = (size_t)g.Nu * g.Nv * g.Nang;
     const size_t nvol = (size_t)g.Nx * g.Ny * g.Nz;
     const size_t slab = (size_t)g.Nx * g.Ny * g.B;
-    if ((rc = dalloc((void **)&c->d_sino, nsino * sizeof(float), "projection stack")) ||
-        (rc = dalloc((void **)&c->d_filt, nsino * sizeof(float), "filtered stack")) ||
-        (rc = dalloc(&c->d_stack, fdk_stack_bytes(c, c->storage, g.Nang), "stored stack")) ||
-        (rc = dalloc((void **)&c->d_scales, g.Nang * sizeof(float), "scales")) ||
-        (rc = dalloc((void **)&c->d_maxabs, g.Nang * sizeof(unsigned), "maxabs"))) {
+    if ((rc = ensure((void **)&c->d_sino, c->cap_sino, nsino * sizeof(float), "projection stack")) ||
+        (rc = ensure((void **)&c->d_filt, c->cap_filt, nsino * sizeof(float), "filtered stack")) ||
+        (rc = ensure(&c->d_stack, c->cap_stack, fdk_stack_bytes(c, c->storage, g.Nang),
+                     "stored stack")) ||
+        (rc = ensure((void **)&c->d_scales, c->cap_scales, g.Nang * sizeof(float), "scales")) ||
+        (rc = ensure((void **)&c->d_maxabs, c->cap_maxabs, g.Nang * sizeof(unsigned), "maxabs"))) {
         free_engine(c);
         return rc;
     }
     if (c->volume_mode == FDK_VOLUME_STREAMED) {
+        dfree(c->d_vol);
+        c->cap_vol = 0;
         c->ring_slices = ring_slices_for(g);
-        rc = dalloc((void **)&c->d_ring, (size_t)c->ring_slices * g.Nx * g.Ny * sizeof(float),
-                    "volume ring");
+        rc = ensure((void **)&c->d_ring, c->cap_ring,
+                    (size_t)c->ring_slices * g.Nx * g.Ny * sizeof(float), "volume ring");
     } else {
-        rc = dalloc((void **)&c->d_vol, nvol * sizeof(float), "volume");
+        dfree(c->d_ring);
+        c->cap_ring = 0;
+        rc = ensure((void **)&c->d_vol, c->cap_vol, nvol * sizeof(float), "volume");
     }
     if (!rc && c->collecting)
-        rc = dalloc((void **)&c->d_partial, slab * sizeof(float), "collect slab");
+        rc = ensure((void **)&c->d_partial, c->cap_partial, slab * sizeof(float), "collect slab");
     if (rc) {
         free_engine(c);
         return rc;
@@ -729,12 +797,12 @@ int fdk_calculate_reconstruction(fdk_ctx *c, fdk_get_block_fn get_block, void *u
     if (!get_block) {
         if (c->volume_mode == FDK_VOLUME_STREAMED)
             return fail(FDK_ERR_INVALID_ARGUMENT, "a streamed volume needs a slab callback");
-        if ((rc = stage_backproject(c, c->storage, c->d_stack, c->d_scales, 0, g.Nang, c->d_vol,
-                                    0, st)))
+        if ((rc = backproject_in_angle_chunks(c, angle_bounds(g.Nang), c->d_vol, 0, st)))
             return rc;
         TKV_HIP(hipEventRecord(c->ev[3], st));
         TKV_HIP(hipEventRecord(c->ev[4], st));
-    } else if ((rc = backproject_and_hand_out(c, get_block, user))) {
+    } else if ((rc = backproject_and_hand_out(c, get_block, user, c->d_stack, c->d_scales, 0,
+                                              g.Nang, 0))) {
         (void)hipStreamSynchronize(c->copy_stream);
         return rc;
     }
@@ -742,6 +810,83 @@ int fdk_calculate_reconstruction(fdk_ctx *c, fdk_get_block_fn get_block, void *u
     TKV_HIP(hipStreamSynchronize(c->copy_stream));
     for (int i = 0; i < 4; ++i)
         TKV_HIP(hipEventElapsedTime(&c->stage_ms[i], c->ev[i], c->ev[i + 1]));
+    return FDK_OK;
+}
+
+// Initialize + CalculateReconstruction with the host stack borrowed for the
+// call: the angles are cut into K chunks; chunk k+1 crosses the host link on
+// the copy stream (a 2-D copy of its [v][a0, a1)[u] runs into a compact
+// [v][nc][u] block) while chunk k is weighted, filtered, packed and
+// backprojected -- the first chunk replacing the volume, the others added onto
+// it -- on the compute stream; the last chunk's backprojection goes through
+// the slab pipeline.  A streamed volume cannot be accumulated over chunks, so
+// it takes one chunk.  The order of host calls matters for pageable memory,
+// whose copies return only when done: each chunk's kernels are queued before
+// the next chunk's copy is issued.
+int fdk_reconstruct_host(fdk_ctx *c, const float *host_sino, fdk_get_block_fn get_block,
+                         void *user)
+{
+    if (!c || !host_sino) return fail(FDK_ERR_INVALID_ARGUMENT, "null argument");
+    if (!get_block && c->volume_mode == FDK_VOLUME_STREAMED)
+        return fail(FDK_ERR_INVALID_ARGUMENT, "a streamed volume needs a slab callback");
+    c->host_sino = nullptr;   // allocate without an upload
+    int rc = fdk_initialize(c);
+    if (rc) return rc;
+    if ((rc = set_device(c))) return rc;
+    const tkv::Geometry &g = c->g;
+    hipStream_t st = c->stream, cs = c->copy_stream;
+    const size_t per_ang = (size_t)g.Nu * g.Nv;
+    const int pitch = tkv::stack_pitch(g.Nu);
+    const size_t stored = (size_t)sample_bytes(c->storage) * pitch * g.Nv;
+    const std::vector<int> ab =
+        c->volume_mode == FDK_VOLUME_STREAMED ? std::vector<int>{0, g.Nang} : angle_bounds(g.Nang);
+    const int K = (int)ab.size() - 1;
+    const size_t host_row = (size_t)g.Nang * g.Nu * sizeof(float);
+    auto upload = [&](int k) -> int {
+        const int a0 = ab[k], nc = ab[k + 1] - a0;
+        const size_t w = (size_t)nc * g.Nu * sizeof(float);
+        TKV_HIP(hipMemcpy2DAsync(c->d_sino + (size_t)a0 * per_ang, w, host_sino + (size_t)a0 * g.Nu,
+                                 host_row, w, g.Nv, hipMemcpyHostToDevice, cs));
+        TKV_HIP(hipEventRecord(c->ev_up[k & 1], cs));
+        return FDK_OK;
+    };
+    if ((rc = tkv::launch_zero_u32(c->d_maxabs, g.Nang, st))) return rc;
+    TKV_HIP(hipEventRecord(c->ev[0], cs));
+    if ((rc = upload(0))) return rc;
+    for (int k = 0; k < K; ++k) {
+        const int a0 = ab[k], nc = ab[k + 1] - a0;
+        float *in = c->d_sino + (size_t)a0 * per_ang, *filt = c->d_filt + (size_t)a0 * per_ang;
+        void *stack = (char *)c->d_stack + (size_t)a0 * stored;
+        float *scales = c->d_scales + a0;
+        TKV_HIP(hipStreamWaitEvent(st, c->ev_up[k & 1], 0));
+        if ((rc = stage_filter(c, FDK_F32, in, filt, nc, 1, 0, c->d_maxabs + a0, st)) ||
+            (rc = tkv::launch_pack(FDK_F32, filt, g.Nu, g.Nv, nc, pitch, c->d_maxabs + a0,
+                                   c->storage, stack, scales, st)))
+            break;
+        if (k + 1 < K || !get_block) {
+            if ((rc = stage_backproject(c, c->storage, stack, scales, a0, nc, c->d_vol, k > 0, st)))
+                break;
+            if (k + 1 < K) {
+                if ((rc = upload(k + 1))) break;
+            } else {
+                TKV_HIP(hipEventRecord(c->ev[3], st));
+                TKV_HIP(hipEventRecord(c->ev[4], st));
+            }
+        } else if ((rc = backproject_and_hand_out(c, get_block, user, stack, scales, a0, nc,
+                                                  K > 1))) {
+            break;
+        }
+    }
+    if (rc) {
+        (void)hipStreamSynchronize(cs);
+        (void)hipStreamSynchronize(st);
+        return rc;
+    }
+    TKV_HIP(hipStreamSynchronize(st));
+    TKV_HIP(hipStreamSynchronize(cs));
+    c->stage_ms[0] = c->stage_ms[1] = 0.f;
+    TKV_HIP(hipEventElapsedTime(&c->stage_ms[2], c->ev[0], c->ev[3]));
+    TKV_HIP(hipEventElapsedTime(&c->stage_ms[3], c->ev[3], c->ev[4]));
     return FDK_OK;
 }
 

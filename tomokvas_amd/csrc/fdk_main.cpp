@@ -66,9 +66,9 @@ int main(int argc, char **argv)
         return 1;
     }
     if (opt.verbose)
-        std::printf("read %.1f MiB, wrote %.1f MiB; engine %.3f s (of which initialise and "
-                    "upload %.3f s), waiting for projections %.3f s, final writes %.3f s\n",
+        std::printf("read %.1f MiB, wrote %.1f MiB; engine %.3f s (upload included), waiting "
+                    "for projections %.3f s, final writes %.3f s\n",
                     st.bytes_read / 1048576.0, st.bytes_written / 1048576.0, st.seconds_engine,
-                    st.seconds_initialize, st.seconds_load_wait, st.seconds_flush);
+                    st.seconds_load_wait, st.seconds_flush);
     return 0;
 }

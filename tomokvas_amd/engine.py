@@ -165,16 +165,17 @@ class FilteredBackProjection:
         self._sino_ref = None     # borrowed only until Initialize returns
         return rc
 
-    def CalculateReconstruction(self) -> int:
+    def _with_sink(self, run) -> int:
+        """run(get_block_fn, user) with the sink of SetReconstructionProperties."""
         sink = self._sink
         if sink is None:
             # no sink: the volume stays in HBM (device_volume_ptr)
-            return self._L.fdk_calculate_reconstruction(self._h, _lib.GET_BLOCK_FN(), None)
+            return run(_lib.GET_BLOCK_FN(), None)
         native = getattr(sink, "native_get_block", None)
         if native is not None:
             # a C writer (fileio.BinSiz): the engine calls it directly
             fn, user = native
-            return self._L.fdk_calculate_reconstruction(self._h, fn, user)
+            return run(fn, user)
         keep = {}
 
         def get_block(user, n, collecting):
@@ -187,10 +188,22 @@ class FilteredBackProjection:
 
         cb = _lib.GET_BLOCK_FN(get_block)
         self._cb = cb
-        rc = self._L.fdk_calculate_reconstruction(self._h, cb, None)
-        if sink is not None and hasattr(sink, "finish"):
+        rc = run(cb, None)
+        if hasattr(sink, "finish"):
             sink.finish()
         return rc
+
+    def CalculateReconstruction(self) -> int:
+        return self._with_sink(
+            lambda fn, user: self._L.fdk_calculate_reconstruction(self._h, fn, user))
+
+    def ReconstructHost(self, sinogram) -> int:
+        """Initialize + CalculateReconstruction with the host stack
+        [Nv][Nang][Nu] uploaded in angle chunks behind the computation (set the
+        sinogram properties with None first)."""
+        arr = np.ascontiguousarray(sinogram, dtype=np.float32)
+        return self._with_sink(
+            lambda fn, user: self._L.fdk_reconstruct_host(self._h, arr.ctypes.data, fn, user))
 
     @staticmethod
     def GetFreeMemInfo(gpu_id: int) -> int:
