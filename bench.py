@@ -147,9 +147,10 @@ def slab_pipeline(fbp, g, hipm, storage, steps):
 
 
 def read_profile(workload: str, storage: str):
-    """(HBM bytes, VALU instructions per voxel-update) of one backprojection launch,
-    measured by rocprofv3 --pmc (profiles/bp_traffic_<workload>.txt, lines
-    'storage fetch_bytes write_bytes valu_per_update'); (None, None) if absent."""
+    """(HBM bytes, VALU instructions per voxel-update) of one backprojection step
+    (its launches summed), measured by rocprofv3 --pmc (profiles/bp_traffic_
+    <workload>.txt, lines 'storage fetch_bytes write_bytes valu_per_update');
+    (None, None) if absent."""
     path = os.path.join(ROOT, "profiles", f"bp_traffic_{workload}.txt")
     if not os.path.exists(path):
         return None, None
@@ -349,12 +350,19 @@ def main():
     achieved = gather / (r["bp_ms"] * 1e-3) / 1e9   # GB/s
     traffic, valu_per_update = read_profile(args.workload, head)
     t_bp = r["bp_ms"] * 1e-3
+    nlaunch = C.c_int()
+    _lib.check(L.fdk_get_angle_chunks(fbp.handle, C.byref(nlaunch)), "angle chunks")
+    # one step backprojects the angle loop in K launches of Nang/K angles each:
+    # per-launch figures are the step's divided by K (achieved is unchanged);
+    # the traffic profile holds the step's bytes (its K dispatches summed)
+    K = nlaunch.value
     roofline = {
         "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-        "kernel": f"bp_kernel<{head}>", "algorithmic_bytes": gather,
+        "frac": achieved / HBM_PEAK_GBS, "traffic": traffic / K if traffic else None,
+        "kernel": f"bp_kernel<{head}>", "algorithmic_bytes": gather / K,
+        "launches_per_step": K, "avg_launch_ms": r["bp_ms"] / K,
         "l2_peak": L2_PEAK_GBS, "frac_l2": achieved / L2_PEAK_GBS,
-        "compulsory_hbm_bytes": g.Nang * g.Nv * g.Nu * s_bytes + 4 * g.Nx * g.Ny * g.Nz,
+        "compulsory_hbm_bytes_per_step": g.Nang * g.Nv * g.Nu * s_bytes + 4 * g.Nx * g.Ny * g.Nz,
         "hbm_traffic_rate_gbs": traffic / t_bp / 1e9 if traffic else None,
     }
     # the L2 figure measured on this device (2 MiB, resident in every XCD's L2)

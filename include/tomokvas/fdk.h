@@ -103,6 +103,11 @@ int fdk_set_storage(fdk_ctx *ctx, int dtype);
  * volume may exceed HBM
  * (kernel.cu:819-888 computes one slab at a time without overlap). */
 int fdk_set_volume_mode(fdk_ctx *ctx, int mode);
+/* launches of one whole-volume backprojection: the angle loop is cut into up
+ * to 8 chunks of >= 100 angles (workgroups of one long launch drift apart in
+ * their progress through the angles; a launch boundary every ~200 angles keeps
+ * the projections they read together) */
+int fdk_get_angle_chunks(fdk_ctx *ctx, int *chunks);
 /* largest backprojection chunk of the slab pipeline, slices (a multiple of
  * the backprojector's z-run for the current storage type) */
 int fdk_get_volume_chunk(fdk_ctx *ctx, int *slices);

@@ -71,6 +71,7 @@ _SIGS = {
     "fdk_probe_l2_read": (_i, [C.c_size_t, C.POINTER(C.c_double)]),
     "fdk_set_volume_mode": (_i, [_vp, _i]),
     "fdk_get_volume_chunk": (_i, [_vp, C.POINTER(_i)]),
+    "fdk_get_angle_chunks": (_i, [_vp, C.POINTER(_i)]),
     # file formats (fdk_io.h)
     "fdk_cra_open": (_i, [C.c_char_p, C.c_char_p, C.POINTER(_vp)]),
     "fdk_cra_close": (None, [_vp]),

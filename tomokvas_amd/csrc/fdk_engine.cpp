@@ -768,6 +768,13 @@ int fdk_set_volume_mode(fdk_ctx *c, int mode)
     return FDK_OK;
 }
 
+int fdk_get_angle_chunks(fdk_ctx *c, int *chunks)
+{
+    if (!c || !chunks) return fail(FDK_ERR_INVALID_ARGUMENT, "null argument");
+    *chunks = (int)angle_bounds(c->g.Nang).size() - 1;
+    return FDK_OK;
+}
+
 int fdk_get_volume_chunk(fdk_ctx *c, int *slices)
 {
     if (!c || !slices) return fail(FDK_ERR_INVALID_ARGUMENT, "null argument");
