@@ -808,10 +808,26 @@ int fdk_calculate_reconstruction(fdk_ctx *c, fdk_get_block_fn get_block, void *u
             return rc;
         TKV_HIP(hipEventRecord(c->ev[3], st));
         TKV_HIP(hipEventRecord(c->ev[4], st));
-    } else if ((rc = backproject_and_hand_out(c, get_block, user, c->d_stack, c->d_scales, 0,
-                                              g.Nang, 0))) {
-        (void)hipStreamSynchronize(c->copy_stream);
-        return rc;
+    } else {
+        // resident: all angle chunks but the last over the whole volume, the
+        // last through the slab pipeline (slabs leave as its z-chunks finish);
+        // streamed: one chunk of every angle through the pipeline
+        const std::vector<int> ab = c->volume_mode == FDK_VOLUME_STREAMED
+                                        ? std::vector<int>{0, g.Nang}
+                                        : angle_bounds(g.Nang);
+        const int K = (int)ab.size() - 1;
+        const size_t stored =
+            (size_t)sample_bytes(c->storage) * tkv::stack_pitch(g.Nu) * g.Nv;
+        if (K > 1 && (rc = backproject_in_angle_chunks(
+                          c, std::vector<int>(ab.begin(), ab.end() - 1), c->d_vol, 0, st)))
+            return rc;
+        const int a0 = ab[K - 1];
+        if ((rc = backproject_and_hand_out(c, get_block, user,
+                                           (const char *)c->d_stack + (size_t)a0 * stored,
+                                           c->d_scales + a0, a0, g.Nang - a0, K > 1))) {
+            (void)hipStreamSynchronize(c->copy_stream);
+            return rc;
+        }
     }
     TKV_HIP(hipStreamSynchronize(st));
     TKV_HIP(hipStreamSynchronize(c->copy_stream));
