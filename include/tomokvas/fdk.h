@@ -104,7 +104,7 @@ int fdk_set_storage(fdk_ctx *ctx, int dtype);
  * (kernel.cu:819-888 computes one slab at a time without overlap). */
 int fdk_set_volume_mode(fdk_ctx *ctx, int mode);
 /* launches of one whole-volume backprojection: the angle loop is cut into up
- * to 8 chunks of >= 100 angles (workgroups of one long launch drift apart in
+ * to 16 chunks of >= 100 angles (workgroups of one long launch drift apart in
  * their progress through the angles; a launch boundary every ~200 angles keeps
  * the projections they read together) */
 int fdk_get_angle_chunks(fdk_ctx *ctx, int *chunks);
@@ -125,7 +125,7 @@ int fdk_calculate_reconstruction(fdk_ctx *ctx, fdk_get_block_fn get_block, void 
 /* extension: fdk_initialize + fdk_calculate_reconstruction in one call with
  * the host stack [v][beta][u] borrowed for its duration (set the sinogram
  * properties with a NULL pointer).  With a resident volume the angles are cut
- * into up to 8 chunks of >= 100 angles and chunk k+1 is uploaded while chunk k
+ * into up to 16 chunks of >= 100 angles and chunk k+1 is uploaded while chunk k
  * is filtered and backprojected, so the host-link transfer hides behind the
  * computation (page-locked host memory transfers fastest).  Stage times:
  * [0], [1] zero, [2] first upload to last backprojection, [3] slab transfers

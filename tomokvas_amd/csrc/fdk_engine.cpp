@@ -358,9 +358,11 @@ int check_angles(fdk_ctx *c, int ang0, int nang)
 // workgroups drift apart in their progress through the angles, so the
 // projections read at any moment spread over many angles; a launch boundary
 // every ~200 angles keeps them together (measured at 1024^3 from 1800
-// projections: 500 ms in one launch, 476 ms in eight).
+// projections, FP16, on one GPU: 519 ms in one launch, 487 ms in 8, 484 in
+// 12, 480 in 16, 487 in 24, 503 in 36 -- launch tails and the volume's
+// read-modify-write grow with the count).
 #ifndef TKV_BP_ANGLE_CHUNKS
-#define TKV_BP_ANGLE_CHUNKS 8
+#define TKV_BP_ANGLE_CHUNKS 16
 #endif
 #ifndef TKV_BP_CHUNK_MIN_ANGLES
 #define TKV_BP_CHUNK_MIN_ANGLES 100
