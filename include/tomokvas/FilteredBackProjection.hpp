@@ -66,6 +66,8 @@ public:
         keep(fdk_set_filter(ctx_, kind, taps, ntaps));
     }
     void SetStorage(int dtype) { keep(fdk_set_storage(ctx_, dtype)); }
+    // FDK_VOLUME_RESIDENT (default) or FDK_VOLUME_STREAMED, before Initialize
+    void SetVolumeMode(int mode) { keep(fdk_set_volume_mode(ctx_, mode)); }
 
     int Initialize()
     {
@@ -77,6 +79,13 @@ public:
     int CalculateReconstruction()
     {
         return report(fdk_calculate_reconstruction(ctx_, get_block_, sink_));
+    }
+    // Initialize + CalculateReconstruction for a caller that keeps the host
+    // stack [v][beta][u] until the call returns (SetSinogramProperties with a
+    // NULL pointer): the upload runs in angle chunks behind the computation.
+    int ReconstructHost(const float *host_sino)
+    {
+        return report(fdk_reconstruct_host(ctx_, host_sino, get_block_, sink_));
     }
     void fShowInfo() const
     {
