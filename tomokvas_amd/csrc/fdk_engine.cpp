@@ -813,8 +813,8 @@ int fdk_calculate_reconstruction(fdk_ctx *c, fdk_get_block_fn get_block, void *u
     } else {
         // resident: all angle chunks but the last over the whole volume, the
         // last through the slab pipeline (slabs leave as its z-chunks finish;
-        // at most 8 chunks, so that the last is long enough to hide most of
-        // the transfers: 568 ms against 583 ms with 16 at 1024^3);
+        // at most 8 chunks, so that the last one lasts long enough to hide
+        // most of the slab transfers);
         // streamed: one chunk of every angle through the pipeline
         const std::vector<int> ab = c->volume_mode == FDK_VOLUME_STREAMED
                                         ? std::vector<int>{0, g.Nang}
