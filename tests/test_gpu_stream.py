@@ -141,3 +141,47 @@ def test_reconstruct_host_overlaps_upload(storage, with_sink):
         _lib.check(fbp.ReconstructHost(s), "ReconstructHost streamed")
         fbp.close()
         assert O.rel_rmse(sink2.volume, want) <= 1e-6
+
+
+def test_reconstruct_host_collecting_over_angle_blocks():
+    # two angle blocks of 250 projections (two chunks each), the second
+    # collecting through the sink, against one pass over all 500
+    N = 500
+    g = scan(64, 48, N, 48, 48, 24, B=8)
+    rng = np.random.default_rng(13)
+    s = rng.standard_normal((g.Nv, g.Nang, g.Nu)).astype(np.float32)
+    want = one_launch(g, s, _lib.F32)
+    sink = VolumeSink(g.Nx, g.Ny, g.Nz, g.B)
+    fbp = None
+    for blk in range(2):
+        gb = scan(64, 48, N // 2, 48, 48, 24, B=8, rot=blk * (N // 2) * g.dbeta)
+        gb.dbeta = g.dbeta
+        part = np.ascontiguousarray(s[:, blk * N // 2:(blk + 1) * N // 2, :])
+        fbp = engine(gb, None, sink, collecting=bool(blk))
+        say(f"reconstruct_host collecting block {blk}")
+        _lib.check(fbp.ReconstructHost(part), "ReconstructHost")
+        fbp.close()
+    e = O.rel_rmse(sink.volume, want)
+    say(f"  two collected blocks vs one pass rel RMSE {e:.2e}")
+    assert e <= 1e-5
+
+
+def test_buffers_kept_across_initialisations():
+    # one context re-initialised for a larger, then a smaller, then the first
+    # geometry (buffers grow once and are kept): each result equals a fresh
+    # context's
+    geoms = [scan(48, 40, 120, 40, 40, 24, B=8), scan(64, 56, 240, 56, 52, 40, B=8),
+             scan(48, 40, 120, 40, 40, 24, B=8)]
+    rng = np.random.default_rng(17)
+    data = [rng.standard_normal((g.Nv, g.Nang, g.Nu)).astype(np.float32) for g in geoms]
+    fresh = [one_launch(g, s, _lib.F16) for g, s in zip(geoms, data)]
+    sink0 = VolumeSink(geoms[0].Nx, geoms[0].Ny, geoms[0].Nz, geoms[0].B)
+    fbp = engine(geoms[0], data[0], sink0, storage=_lib.F16)
+    for i, (g, s) in enumerate(zip(geoms, data)):
+        sink = VolumeSink(g.Nx, g.Ny, g.Nz, g.B)
+        g.apply(fbp, s, sink)
+        say(f"re-initialised context, geometry {i}")
+        _lib.check(fbp.Initialize(), "Initialize")
+        _lib.check(fbp.CalculateReconstruction(), "CalculateReconstruction")
+        np.testing.assert_array_equal(sink.volume, fresh[i])
+    fbp.close()
