@@ -434,6 +434,10 @@ int backproject_and_hand_out(fdk_ctx *c, fdk_get_block_fn get_block, void *user,
     const int zr = tkv::bp_zrun(c->storage);
     if (C < 1 || (wraps && 2 * C + g.B > W))
         return fail(FDK_ERR_INVALID_ARGUMENT, "slab pipeline: volume buffer too small");
+    // a ring slot holds other slices before its chunk is backprojected: only a
+    // resident volume can be added onto
+    if (accumulate && streamed)
+        return fail(FDK_ERR_INVALID_ARGUMENT, "slab pipeline: cannot accumulate a streamed volume");
     const std::vector<int> zb = chunk_bounds(g.Nz, C, zr);
     const int nchunk = (int)zb.size() - 1;
     const int nslab = tkv::slab_count(g.Nz, g.B);
