@@ -185,3 +185,18 @@ def test_buffers_kept_across_initialisations():
         _lib.check(fbp.CalculateReconstruction(), "CalculateReconstruction")
         np.testing.assert_array_equal(sink.volume, fresh[i])
     fbp.close()
+
+
+@pytest.mark.parametrize("storage", [_lib.F32, _lib.F16])
+def test_streamed_chunks_cut_their_angles(storage):
+    # 240 angles: every chunk of a streamed volume (80 slices in a ring of 70,
+    # slab 6) is backprojected in two angle launches, as the resident volume
+    # is -- the same sums in the same order, so equal bit for bit
+    g = scan(48, 40, 240, 48, 44, 80, B=6)
+    rng = np.random.default_rng(23)
+    s = rng.standard_normal((g.Nv, g.Nang, g.Nu)).astype(np.float32)
+    want = one_launch(g, s, storage)
+    say(f"streamed 240 angles storage={storage}")
+    sink, chunk, times, ptr = piped(g, s, storage, _lib.VOLUME_STREAMED)
+    assert ptr == 0
+    np.testing.assert_array_equal(sink.volume, want)
