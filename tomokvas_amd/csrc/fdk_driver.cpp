@@ -33,6 +33,7 @@
 #include <memory>
 #include <sstream>
 #include <string>
+#include <system_error>
 #include <thread>
 
 using tkv::fail;
@@ -356,7 +357,11 @@ extern "C" int fdk_reconstruct_file(const char *cra_path, const fdk_setting *set
             Block &nxt = blk[(i + 1) & 1];
             nxt.scan = (i + 1) / angle_blocks;
             nxt.angle = (i + 1) % angle_blocks;
-            loader = std::thread(load_block, cra.p, proj_floats, &nxt);
+            try {
+                loader = std::thread(load_block, cra.p, proj_floats, &nxt);
+            } catch (const std::system_error &) {
+                load_block(cra.p, proj_floats, &nxt);   // no thread available: load now
+            }
         }
         const int scan = cur.scan, k = cur.angle;
         const float centre = (float)m.centre_of_rotation[scan];

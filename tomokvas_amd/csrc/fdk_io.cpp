@@ -284,7 +284,13 @@ extern "C" int fdk_cra_load(const fdk_cra *cra, float *dst)
         }
     };
     std::vector<std::thread> pool;
-    for (int t = 1; t < nthreads; ++t) pool.emplace_back(work, t);
+    for (int t = 1; t < nthreads; ++t) {
+        try {
+            pool.emplace_back(work, t);
+        } catch (const std::system_error &) {
+            work(t);   // no thread available: this share now
+        }
+    }
     work(0);
     for (auto &th : pool) th.join();
     close(fd);

@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <system_error>
 #include <thread>
 #include <vector>
 
@@ -259,10 +260,15 @@ extern "C" int fdk_tiff_export_device(const float *d_vol, int x, int y, int z, c
         if (wrc) break;
         e = hipMemcpy(h, d_u16, (size_t)nz * plane * 2, hipMemcpyDeviceToHost);
         if (e != hipSuccess) break;
-        writer = std::thread([&, h, nz, z0]() {
+        auto write = [&, h, nz, z0]() {
             wrc = write_slices(h, x, y, nz, dir, first_index + z0);
             if (wrc) werr = fdk_last_error_string();
-        });
+        };
+        try {
+            writer = std::thread(write);
+        } catch (const std::system_error &) {
+            write();   // no thread available: write now
+        }
     }
     join();
     (void)hipFree(d_u16);
