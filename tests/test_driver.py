@@ -85,6 +85,15 @@ def test_command_line_usage_and_errors(tmp_path):
     assert r.returncode == 1 and "none.CRA" in r.stderr
     found, s = LoadSetting(str(tmp_path / "setting.xml"))
     assert found and as_tuple(s) == as_tuple(default_setting())
+    # every option is accepted (the run then fails on the missing file, not on usage)
+    for opts in (["--streamed"], ["--resident"], ["--storage", "bf16", "--shepp-logan"],
+                 ["--angle-block-mib", "64", "--setting", str(tmp_path / "s2.xml")]):
+        r = subprocess.run([EXE, "--quiet", *opts, str(tmp_path / "none.CRA")],
+                           capture_output=True, text=True, cwd=tmp_path)
+        assert r.returncode == 1 and "none.CRA" in r.stderr, opts
+    assert os.path.exists(tmp_path / "s2.xml")     # --setting names the file written
+    r = subprocess.run([EXE, "--bogus", "x.CRA"], capture_output=True, text=True, cwd=tmp_path)
+    assert r.returncode == 255 and "usage:" in r.stderr
 
 
 def test_driver_rejects_bad_setting(tmp_path):
