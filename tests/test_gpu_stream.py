@@ -200,3 +200,31 @@ def test_streamed_chunks_cut_their_angles(storage):
     sink, chunk, times, ptr = piped(g, s, storage, _lib.VOLUME_STREAMED)
     assert ptr == 0
     np.testing.assert_array_equal(sink.volume, want)
+
+
+def test_setters_after_initialize_require_it_again():
+    # a size-changing setter after Initialize must not let CalculateReconstruction
+    # run with buffers sized for the old geometry
+    g = scan(48, 40, 30, 40, 40, 24, B=8)
+    s = np.random.default_rng(29).standard_normal((g.Nv, g.Nang, g.Nu)).astype(np.float32)
+    sink = VolumeSink(g.Nx, g.Ny, g.Nz, g.B)
+    fbp = engine(g, s, sink)
+    _lib.check(fbp.Initialize(), "Initialize")
+    _lib.check(fbp.CalculateReconstruction(), "CalculateReconstruction")
+    big = scan(48, 40, 30, 64, 64, 48, B=8)
+    sink2 = VolumeSink(big.Nx, big.Ny, big.Nz, big.B)
+    _lib.check(fbp.SetReconstructionProperties(sink2, big.Nx, big.cx, big.Ny, big.cy, big.Nz,
+                                               big.cz, big.rot))
+    assert fbp.CalculateReconstruction() == _lib.FDK_ERR_NOT_INITIALIZED
+    _lib.check(fbp.SetCollecting(True))
+    assert fbp.CalculateReconstruction() == _lib.FDK_ERR_NOT_INITIALIZED
+    _lib.check(fbp.SetCollecting(False))
+    _lib.check(fbp.SetVolumeMode(_lib.VOLUME_STREAMED))
+    assert fbp.CalculateReconstruction() == _lib.FDK_ERR_NOT_INITIALIZED
+    _lib.check(fbp.SetVolumeMode(_lib.VOLUME_RESIDENT))
+    # initialising again keeps the projections already on the device
+    _lib.check(fbp.Initialize(), "Initialize again")
+    _lib.check(fbp.CalculateReconstruction(), "CalculateReconstruction again")
+    fbp.close()
+    want = one_launch(big, s, _lib.F32)
+    np.testing.assert_array_equal(sink2.volume, want)

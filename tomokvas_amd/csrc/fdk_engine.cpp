@@ -598,6 +598,7 @@ int fdk_set_block_size(fdk_ctx *c, int slices)
     if (!c || slices < 1) return fail(FDK_ERR_INVALID_ARGUMENT, "block size must be positive");
     c->g.B = slices;
     c->prepared = false;
+    c->initialized = false;   // sizes or tables change: fdk_initialize again
     return FDK_OK;
 }
 
@@ -611,6 +612,7 @@ int fdk_set_allocation_limit(fdk_ctx *c, long long bytes)
 int fdk_set_collecting(fdk_ctx *c, int collecting)
 {
     if (!c) return fail(FDK_ERR_INVALID_ARGUMENT, "null context");
+    if (collecting && !c->d_partial) c->initialized = false;   // needs its collect slab
     c->collecting = collecting ? 1 : 0;
     return FDK_OK;
 }
@@ -631,6 +633,7 @@ int fdk_set_axis_direction(fdk_ctx *c, int fx, int fy, int fz)
     c->g.flipY = fy ? 1 : 0;
     c->g.flipZ = fz ? 1 : 0;
     c->prepared = false;
+    c->initialized = false;   // sizes or tables change: fdk_initialize again
     return FDK_OK;
 }
 
@@ -649,6 +652,7 @@ int fdk_set_sinogram_properties(fdk_ctx *c, const float *sino, int Nu, float cX,
     c->g.tau = sampling;
     c->have_sino = true;
     c->prepared = false;
+    c->initialized = false;   // sizes or tables change: fdk_initialize again
     return FDK_OK;
 }
 
@@ -665,6 +669,7 @@ int fdk_set_reconstruction_properties(fdk_ctx *c, int Nx, float cx, int Ny, floa
     c->g.rot = rot;
     c->have_rec = true;
     c->prepared = false;
+    c->initialized = false;   // sizes or tables change: fdk_initialize again
     return FDK_OK;
 }
 
@@ -673,6 +678,7 @@ int fdk_set_scale_xyz(fdk_ctx *c, float scale)
     if (!c) return fail(FDK_ERR_INVALID_ARGUMENT, "null context");
     c->g.scale = scale;
     c->prepared = false;
+    c->initialized = false;   // sizes or tables change: fdk_initialize again
     return FDK_OK;
 }
 
@@ -687,6 +693,7 @@ int fdk_set_filter(fdk_ctx *c, int kind, const double *taps, int ntaps)
     c->user_taps.assign(taps && kind == FDK_FILTER_USER ? taps : nullptr,
                         taps && kind == FDK_FILTER_USER ? taps + ntaps : nullptr);
     c->prepared = false;
+    c->initialized = false;   // sizes or tables change: fdk_initialize again
     return FDK_OK;
 }
 
@@ -785,6 +792,7 @@ int fdk_set_volume_mode(fdk_ctx *c, int mode)
     if (mode != FDK_VOLUME_RESIDENT && mode != FDK_VOLUME_STREAMED)
         return fail(FDK_ERR_INVALID_ARGUMENT, "volume mode must be FDK_VOLUME_RESIDENT or "
                                               "FDK_VOLUME_STREAMED");
+    if (mode != c->volume_mode) c->initialized = false;   // other volume buffers
     c->volume_mode = mode;
     return FDK_OK;
 }
