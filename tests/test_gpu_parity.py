@@ -1,6 +1,6 @@
 """GPU parity of the HIP FDK path against the CPU oracle (through the C ABI).
 
-Bars: FP32 path vs the FP32 mirror <= 1e-5 relative RMSE;
+Bars: FP32 path vs the FP32 mirror <= 1e-5 relative RMSE (the FP32 parity bar);
 stage-level FP32 kernels much tighter; FP64 instantiations vs the FP64 oracle
 <= 1e-12; FP16/BF16 storage <= 1e-3 vs the FP64 oracle.
 """
