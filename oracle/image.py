@@ -5,7 +5,7 @@ ConvertToTiff (FBPG/image_file.cpp:104-110): u = (unsigned short)
 (65535 * (x - min) / (max - min)) in FP64 with the volume's range; pinned in
 tests/test_tiff.py by the closed forms min -> 0, max -> 65535 and the
 midpoint -> 32767.  Values the reference leaves undefined (outside the range,
-NaN, max == min) follow the rule of this build: clamp, 0, all zeros.
+NaN, max == min) follow these rules: clamp, 0, all zeros.
 """
 import numpy as np
 
