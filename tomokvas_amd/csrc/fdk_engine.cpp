@@ -44,7 +44,8 @@ struct fdk_ctx {
     // volume is backprojected on stream
     int volume_mode = FDK_VOLUME_RESIDENT;
     hipStream_t copy_stream = nullptr;
-    hipEvent_t ev_bp[2] = {}, ev_cp[2] = {}, ev_up[2] = {};
+    hipStream_t prep_stream = nullptr;   // fdk_reconstruct_host: filter and pack of a chunk
+    hipEvent_t ev_bp[2] = {}, ev_cp[2] = {}, ev_up[2] = {}, ev_f[2] = {};
     int ring_slices = 0;     // streamed volume: slices held in HBM (circular)
 
     // tables (prepared)
@@ -527,11 +528,13 @@ int fdk_create(int gpu_id, fdk_ctx **out)
     hipError_t e = hipSetDevice(gpu_id);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->prep_stream, hipStreamNonBlocking);
     for (int i = 0; i < 5 && e == hipSuccess; ++i) e = hipEventCreate(&c->ev[i]);
     for (int i = 0; i < 2 && e == hipSuccess; ++i) {
         e = hipEventCreateWithFlags(&c->ev_bp[i], hipEventDisableTiming);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_cp[i], hipEventDisableTiming);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_up[i], hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_f[i], hipEventDisableTiming);
     }
     if (e != hipSuccess) {
         fdk_destroy(c);
@@ -547,6 +550,7 @@ int fdk_destroy(fdk_ctx *c)
     (void)hipSetDevice(c->dev);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->copy_stream) (void)hipStreamSynchronize(c->copy_stream);
+    if (c->prep_stream) (void)hipStreamSynchronize(c->prep_stream);
     free_engine(c);
     free_tables(c);
     for (auto &e : c->ev)
@@ -555,9 +559,11 @@ int fdk_destroy(fdk_ctx *c)
         if (c->ev_bp[i]) (void)hipEventDestroy(c->ev_bp[i]);
         if (c->ev_cp[i]) (void)hipEventDestroy(c->ev_cp[i]);
         if (c->ev_up[i]) (void)hipEventDestroy(c->ev_up[i]);
+        if (c->ev_f[i]) (void)hipEventDestroy(c->ev_f[i]);
     }
     if (c->stream) (void)hipStreamDestroy(c->stream);
     if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
+    if (c->prep_stream) (void)hipStreamDestroy(c->prep_stream);
     delete c;
     return FDK_OK;
 }
@@ -873,8 +879,9 @@ int fdk_calculate_reconstruction(fdk_ctx *c, fdk_get_block_fn get_block, void *u
 // [v][nc][u] block) while chunk k is weighted, filtered, packed and
 // backprojected -- the first chunk replacing the volume, the others added onto
 // it -- on the compute stream; the last chunk's backprojection goes through
-// the slab pipeline.  A streamed volume cannot be accumulated over chunks, so
-// it takes one chunk.  The order of host calls matters for pageable memory,
+// the slab pipeline.  Filter and pack run on a third stream, so chunk k+1's
+// overlap chunk k's backprojection.  A streamed volume cannot be accumulated
+// over chunks, so it takes one chunk.  The order of host calls matters for pageable memory,
 // whose copies return only when done: each chunk's kernels are queued before
 // the next chunk's copy is issued.
 int fdk_reconstruct_host(fdk_ctx *c, const float *host_sino, fdk_get_block_fn get_block,
@@ -904,7 +911,8 @@ int fdk_reconstruct_host(fdk_ctx *c, const float *host_sino, fdk_get_block_fn ge
         TKV_HIP(hipEventRecord(c->ev_up[k & 1], cs));
         return FDK_OK;
     };
-    if ((rc = tkv::launch_zero_u32(c->d_maxabs, g.Nang, st))) return rc;
+    hipStream_t ps = c->prep_stream;
+    if ((rc = tkv::launch_zero_u32(c->d_maxabs, g.Nang, ps))) return rc;
     TKV_HIP(hipEventRecord(c->ev[0], cs));
     if ((rc = upload(0))) return rc;
     for (int k = 0; k < K; ++k) {
@@ -912,11 +920,16 @@ int fdk_reconstruct_host(fdk_ctx *c, const float *host_sino, fdk_get_block_fn ge
         float *in = c->d_sino + (size_t)a0 * per_ang, *filt = c->d_filt + (size_t)a0 * per_ang;
         void *stack = (char *)c->d_stack + (size_t)a0 * stored;
         float *scales = c->d_scales + a0;
-        TKV_HIP(hipStreamWaitEvent(st, c->ev_up[k & 1], 0));
-        if ((rc = stage_filter(c, FDK_F32, in, filt, nc, 1, 0, c->d_maxabs + a0, st)) ||
+        // the chunk's filter and pack on the prep stream once it has landed,
+        // so they can run while the previous chunk is backprojected; its
+        // backprojection on the compute stream once they are done
+        TKV_HIP(hipStreamWaitEvent(ps, c->ev_up[k & 1], 0));
+        if ((rc = stage_filter(c, FDK_F32, in, filt, nc, 1, 0, c->d_maxabs + a0, ps)) ||
             (rc = tkv::launch_pack(FDK_F32, filt, g.Nu, g.Nv, nc, pitch, c->d_maxabs + a0,
-                                   c->storage, stack, scales, st)))
+                                   c->storage, stack, scales, ps)))
             break;
+        TKV_HIP(hipEventRecord(c->ev_f[k & 1], ps));
+        TKV_HIP(hipStreamWaitEvent(st, c->ev_f[k & 1], 0));
         if (k + 1 < K || !get_block) {
             if ((rc = stage_backproject(c, c->storage, stack, scales, a0, nc, c->d_vol, k > 0, st)))
                 break;
@@ -933,11 +946,13 @@ int fdk_reconstruct_host(fdk_ctx *c, const float *host_sino, fdk_get_block_fn ge
     }
     if (rc) {
         (void)hipStreamSynchronize(cs);
+        (void)hipStreamSynchronize(ps);
         (void)hipStreamSynchronize(st);
         return rc;
     }
     TKV_HIP(hipStreamSynchronize(st));
     TKV_HIP(hipStreamSynchronize(cs));
+    TKV_HIP(hipStreamSynchronize(ps));
     c->stage_ms[0] = c->stage_ms[1] = 0.f;
     TKV_HIP(hipEventElapsedTime(&c->stage_ms[2], c->ev[0], c->ev[3]));
     TKV_HIP(hipEventElapsedTime(&c->stage_ms[3], c->ev[3], c->ev[4]));
