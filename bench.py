@@ -99,6 +99,70 @@ def end_to_end(fbp, g, hipm, storage, steps):
             "rel_rmse_overlapped_vs_serial": math.sqrt(out3[0] / out3[1])}
 
 
+def rsq_leg(hipm, steps):
+    """Scanco RSQ preprocessing on the GPU (fdk_rsq_process): one detector block
+    of 248 sinograms of 2000 projections x 1024 columns (the scanner's layout,
+    1.02 GB of 16-bit counts) rebinned to full fan (2000 x 2005) in FP32 --
+    an HBM-bound pass (each raw sample read at least once, each output written
+    once)."""
+    L = _lib.load()
+    p = _lib.RsqParams()
+    _lib.check(L.fdk_rsq_default_params(C.byref(p)), "rsq params")
+    p.complete, p.invert = 1, 1
+    n = 248
+    sx_cpl, sy_cpl = 2 * p.sin_x - 1 - 2 * p.axis_move, p.sin_y
+    rng = np.random.default_rng(3)
+    raw = rng.integers(200, 30000, (n, p.sin_y + 2, p.sin_x), dtype=np.uint16)
+    raw[:, 0, :] = 100
+    raw[:, 1, :] = 31000
+    d_raw, d_out = C.c_void_p(), C.c_void_p()
+    out_bytes = n * sy_cpl * sx_cpl * 4
+    assert hipm.hipMalloc(C.byref(d_raw), raw.nbytes) == 0
+    assert hipm.hipMalloc(C.byref(d_out), out_bytes) == 0
+    assert hipm.hipMemcpy(d_raw, C.c_void_p(raw.ctypes.data), raw.nbytes, 1) == 0
+    ts = []
+    for _ in range(steps + 1):
+        hipm.hipDeviceSynchronize()
+        t0 = time.perf_counter()
+        _lib.check(L.fdk_rsq_process(C.byref(p), 1.1, 0.1, d_raw, n, _lib.F32, d_out, None), "rsq")
+        hipm.hipDeviceSynchronize()
+        ts.append(time.perf_counter() - t0)
+    hipm.hipFree(d_raw)
+    hipm.hipFree(d_out)
+    t = float(np.mean(ts[1:]))
+    alg = raw.nbytes + out_bytes
+    return {"sinograms": n, "raw_bytes": raw.nbytes, "output_bytes": out_bytes, "ms": 1e3 * t,
+            "algorithmic_gbs": alg / t / 1e9, "frac_hbm": alg / t / 1e9 / HBM_PEAK_GBS}
+
+
+def tiff_leg(fbp, g, hipm, folder):
+    """TIFF export of the device volume (fdk_tiff_export_device): min/max and
+    16-bit scaling on the GPU, one file per slice into `folder`."""
+    L = _lib.load()
+    os.makedirs(folder, exist_ok=True)
+    nvol = g.Nx * g.Ny * g.Nz
+    dvol = C.c_void_p(fbp.device_volume_ptr())
+    mm = (C.c_double * 2)()
+    hipm.hipDeviceSynchronize()
+    t0 = time.perf_counter()
+    _lib.check(L.fdk_volume_minmax(dvol, nvol, mm, None), "minmax")
+    t_mm = time.perf_counter() - t0
+    d_u16 = C.c_void_p()
+    assert hipm.hipMalloc(C.byref(d_u16), nvol * 2) == 0
+    hipm.hipDeviceSynchronize()
+    t0 = time.perf_counter()
+    _lib.check(L.fdk_volume_to_u16(dvol, nvol, mm[0], mm[1], d_u16, None), "to_u16")
+    hipm.hipDeviceSynchronize()
+    t_u16 = time.perf_counter() - t0
+    hipm.hipFree(d_u16)
+    t0 = time.perf_counter()
+    _lib.check(L.fdk_tiff_export_device(dvol, g.Nx, g.Ny, g.Nz, folder.encode(), 0, mm), "export")
+    t_all = time.perf_counter() - t0
+    return {"slices": g.Nz, "minmax_ms": 1e3 * t_mm, "minmax_gbs": 4 * nvol / t_mm / 1e9,
+            "to_u16_ms": 1e3 * t_u16, "to_u16_gbs": 6 * nvol / t_u16 / 1e9,
+            "export_s": t_all, "files_bytes": 2 * nvol}
+
+
 def slab_pipeline(fbp, g, hipm, storage, steps):
     """FDK with every slab handed to the host (page-locked memory, the writer's
     case): the backprojection runs in chunks while the previous chunk's slabs
@@ -243,6 +307,10 @@ def main():
     ap.add_argument("--workload", default="cfg3")
     ap.add_argument("--storages", default="f32,f16,bf16")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--rsq", action="store_true",
+                    help="also time the RSQ preprocessing (248 sinograms of 2000 x 1024)")
+    ap.add_argument("--tiff", metavar="DIR",
+                    help="also export the FP16-store volume as TIFF slices into DIR")
     ap.add_argument("--no-end-to-end", action="store_true",
                     help="leave out the runs that start from projections in host memory")
     ap.add_argument("--no-slab-pipeline", action="store_true",
@@ -312,7 +380,7 @@ def main():
         cmp = None
         if st == _lib.F32:
             assert hipm.hipMemcpy(ref, C.c_void_p(dvol), nvol * 4, 3) == 0
-        else:
+        elif _lib.F32 in order:           # FP32 ran first: compare with its volume
             out3 = (C.c_double * 3)()
             _lib.check(L.fdk_compare(C.c_void_p(dvol), ref, nvol, out3, None), "compare")
             cmp = math.sqrt(out3[0] / out3[1])
@@ -403,6 +471,17 @@ def main():
         out["cpu_baseline"] = cpu
         out["rel_rmse"].update(errs)
         log(f"cpu leg {time.perf_counter() - t0:.1f} s: {cpu['value']:.3f} GUPS, errors {errs}")
+    if args.tiff:
+        _lib.check(fbp.SetStorage(_lib.F16 if "f16" in res else order[0]), "SetStorage")
+        _lib.check(L.fdk_calculate_reconstruction(fbp.handle, _lib.GET_BLOCK_FN(), None), "tiff fdk")
+        out["tiff"] = tl = tiff_leg(fbp, g, hipm, args.tiff)
+        log(f"tiff: minmax {tl['minmax_ms']:.2f} ms ({tl['minmax_gbs']:.0f} GB/s), 16-bit "
+            f"scaling {tl['to_u16_ms']:.2f} ms ({tl['to_u16_gbs']:.0f} GB/s), {tl['slices']} "
+            f"files in {tl['export_s']:.2f} s")
+    if args.rsq:
+        out["rsq"] = rq = rsq_leg(hipm, args.steps)
+        log(f"rsq: {rq['sinograms']} sinograms in {rq['ms']:.2f} ms "
+            f"({rq['algorithmic_gbs']:.0f} GB/s, {100 * rq['frac_hbm']:.0f} % of HBM)")
     if not args.no_end_to_end:
         e2e_storage = _lib.F16 if "f16" in res else order[0]
         t0 = time.perf_counter()

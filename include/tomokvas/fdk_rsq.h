@@ -65,7 +65,8 @@ int fdk_rsq_sizes(const fdk_rsq *r, const fdk_rsq_params *p, int *num_sinograms,
 int fdk_rsq_load(const fdk_rsq *r, const fdk_rsq_params *p, int first, int last, uint16_t *dst);
 /* ProccessAndCopy (rsq_load.cpp:270-297) on the GPU: d_raw as fdk_rsq_load
  * writes it, d_out [nsin][sin_y_cpl][sin_x_cpl] of out_dtype (FDK_F32 or
- * FDK_F64), on `stream` (NULL: the default stream).  Angles that the
+ * FDK_F64), on `stream` (NULL: the default stream); returns when it is done
+ * (the per-column atan table it uploads is freed then).  Angles that the
  * reference would read outside the scan (delta_fi other than 2 pi / sin_y)
  * wrap around it. */
 int fdk_rsq_process(const fdk_rsq_params *p, double bh_a, double bh_b, const uint16_t *d_raw,

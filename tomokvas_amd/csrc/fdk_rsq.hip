@@ -6,8 +6,9 @@
 // correction and the beam-hardening line on each, and for the full-fan
 // rebinning the linear interpolation between the two angle rows of the
 // redundant ray.  All arithmetic is FP64 in the reference's order of
-// operations (no contraction into FMAs); log and atan are the device's FP64
-// functions.
+// operations (no contraction into FMAs); atan(t / distance) comes from a
+// per-column table computed on the host, as the reference does (atan_vect,
+// rsq_load.cpp:329-333); log is the device's FP64 function.
 #include "tomokvas/fdk.h"
 #include "tomokvas/fdk_rsq.h"
 #include "fdk_internal.hpp"
@@ -19,6 +20,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #pragma clang fp contract(off)
 
@@ -33,6 +35,7 @@ constexpr int kBhOffset = 3432, kBhBytes = 152;
 struct RsqArgs {
     const uint16_t *raw;   // [n][sin_y + 2][sin_x]
     void *out;             // [n][sy][sx]
+    const double *at;      // atan(t / distance) for t = -1 .. -(sin_x-1-axis_move), at[-t]
     int n, sin_x, sin_y, sx, sy, axis_move, start, flip, invert, complete;
     double A, B, distance, delta_fi;
 };
@@ -71,7 +74,7 @@ __global__ void __launch_bounds__(256) rsq_kernel(RsqArgs a)
                 v = line_integral(a, blk, f, a.axis_move + c - c0);
             } else {
                 const int t = c - c0;                    // < 0
-                const double at = atan(t / a.distance);
+                const double at = a.at[-t];              // atan(t / distance), host table
                 const double fi = f * a.delta_fi;
                 double nf = (fi - 2 * at - kPi) / a.delta_fi;
                 if (nf > a.sin_y - 1 || nf < 0) nf = (fi - 2 * at + kPi) / a.delta_fi;
@@ -291,10 +294,24 @@ extern "C" int fdk_rsq_process(const fdk_rsq_params *p, double bh_a, double bh_b
     if (total == 0) return FDK_OK;
     const unsigned blocks = (unsigned)std::min<size_t>((total + 255) / 256, 65536);
     hipStream_t st = (hipStream_t)stream;
-    if (out_dtype == FDK_F64)
-        hipLaunchKernelGGL(rsq_kernel<double>, dim3(blocks), dim3(256), 0, st, a);
-    else
-        hipLaunchKernelGGL(rsq_kernel<float>, dim3(blocks), dim3(256), 0, st, a);
-    TKV_HIP(hipGetLastError());
+    // atan table, index -t for t = -1 .. -(sin_x - 1 - axis_move) (entry 0 unused)
+    const int nt = p->sin_x - p->axis_move;
+    std::vector<double> at(nt, 0.0);
+    for (int t = -p->sin_x + 1 + p->axis_move; t < 0; ++t) at[-t] = std::atan(t / p->distance);
+    double *d_at = nullptr;
+    TKV_HIP(hipMalloc((void **)&d_at, nt * sizeof(double)));
+    hipError_t e = hipMemcpyAsync(d_at, at.data(), nt * sizeof(double), hipMemcpyHostToDevice, st);
+    a.at = d_at;
+    if (e == hipSuccess) {
+        if (out_dtype == FDK_F64)
+            hipLaunchKernelGGL(rsq_kernel<double>, dim3(blocks), dim3(256), 0, st, a);
+        else
+            hipLaunchKernelGGL(rsq_kernel<float>, dim3(blocks), dim3(256), 0, st, a);
+        e = hipGetLastError();
+    }
+    // the table lives until the kernel is done
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(d_at);
+    if (e != hipSuccess) return tkv::hip_fail(e, "rsq");
     return FDK_OK;
 }
