@@ -102,9 +102,10 @@ def end_to_end(fbp, g, hipm, storage, steps):
 def rsq_leg(hipm, steps):
     """Scanco RSQ preprocessing on the GPU (fdk_rsq_process): one detector block
     of 248 sinograms of 2000 projections x 1024 columns (the scanner's layout,
-    1.02 GB of 16-bit counts) rebinned to full fan (2000 x 2005) in FP32 --
-    an HBM-bound pass (each raw sample read at least once, each output written
-    once)."""
+    1.02 GB of 16-bit counts) rebinned to full fan (2000 x 2005) in FP32.  The
+    algorithmic bytes are each raw sample read once and each output sample
+    written once; the work is one FP64 log per raw sample and the rebinning of
+    the redundant half from the FP64 line integrals."""
     L = _lib.load()
     p = _lib.RsqParams()
     _lib.check(L.fdk_rsq_default_params(C.byref(p)), "rsq params")

@@ -65,10 +65,13 @@ int fdk_rsq_sizes(const fdk_rsq *r, const fdk_rsq_params *p, int *num_sinograms,
 int fdk_rsq_load(const fdk_rsq *r, const fdk_rsq_params *p, int first, int last, uint16_t *dst);
 /* ProccessAndCopy (rsq_load.cpp:270-297) on the GPU: d_raw as fdk_rsq_load
  * writes it, d_out [nsin][sin_y_cpl][sin_x_cpl] of out_dtype (FDK_F32 or
- * FDK_F64), on `stream` (NULL: the default stream); returns when it is done
- * (the per-column atan table it uploads is freed then).  Angles that the
- * reference would read outside the scan (delta_fi other than 2 pi / sin_y)
- * wrap around it. */
+ * FDK_F64), on `stream` (NULL: the default stream).  Without completing it
+ * only queues its kernels; with completing (the full-fan rebinning) it
+ * returns when they are done and frees what it allocated for them: the
+ * per-column atan table and an FP64 line-integral scratch of at most 64 MiB
+ * (or one sinogram, sin_y x (sin_x - axis_move) values, when that is larger).
+ * At most 65535 projections per sinogram.  Angles that the reference would
+ * read outside the scan (delta_fi other than 2 pi / sin_y) wrap around it. */
 int fdk_rsq_process(const fdk_rsq_params *p, double bh_a, double bh_b, const uint16_t *d_raw,
                     int nsin, int out_dtype, void *d_out, void *stream);
 

@@ -1,14 +1,19 @@
 // Scanco RSQ raw projections: header on the host, preprocessing on the GPU.
 //
 // The raw sinograms stay 16-bit until they are on the device (a quarter of the
-// bytes of the reference's FP64 sinograms cross the host link); one kernel then
-// forms every output sample from at most two raw samples: flat/dark
-// correction and the beam-hardening line on each, and for the full-fan
-// rebinning the linear interpolation between the two angle rows of the
-// redundant ray.  All arithmetic is FP64 in the reference's order of
-// operations (no contraction into FMAs); atan(t / distance) comes from a
-// per-column table computed on the host, as the reference does (atan_vect,
-// rsq_load.cpp:329-333); log is the device's FP64 function.
+// bytes of the reference's FP64 sinograms cross the host link).  Every output
+// sample comes from at most two raw samples: flat/dark correction and the
+// beam-hardening line on each, and for the full-fan rebinning the linear
+// interpolation between the two angle rows of the redundant ray.  The FP64
+// log dominates the cost, so the rebinning runs in two passes over groups of
+// sinograms: the line integrals of the raw samples it reads, once each, into
+// an FP64 scratch small enough to stay in the last-level cache, then the
+// rebinned rows from them (the same FP64 values the single-pass form would
+// compute, so the result does not depend on the grouping).  All arithmetic is
+// FP64 in the reference's order of operations (no contraction into FMAs);
+// atan(t / distance) comes from a per-column table computed on the host, as
+// the reference does (atan_vect, rsq_load.cpp:329-333); log is the device's
+// FP64 function.
 #include "tomokvas/fdk.h"
 #include "tomokvas/fdk_rsq.h"
 #include "fdk_internal.hpp"
@@ -16,6 +21,7 @@
 #include <hip/hip_runtime.h>
 #include <sys/types.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -31,12 +37,15 @@ namespace {
 constexpr double kPi = 3.14159265358979323846;
 constexpr int kHeaderBytes = 512;     // sizeof(ima_data_type)
 constexpr int kBhOffset = 3432, kBhBytes = 152;
+// line-integral scratch of one group of sinograms in the full-fan rebinning
+// (one sinogram when a single one is larger)
+constexpr size_t kGroupBytes = size_t(64) << 20;
 
 struct RsqArgs {
     const uint16_t *raw;   // [n][sin_y + 2][sin_x]
     void *out;             // [n][sy][sx]
     const double *at;      // atan(t / distance) for t = -1 .. -(sin_x-1-axis_move), at[-t]
-    int n, sin_x, sin_y, sx, sy, axis_move, start, flip, invert, complete;
+    int sin_x, sin_y, sx, sy, axis_move, start, flip, invert, complete;
     double A, B, distance, delta_fi;
 };
 
@@ -51,50 +60,76 @@ __device__ double line_integral(const RsqArgs &a, const uint16_t *blk, int y, in
     return a.invert ? -v : v;
 }
 
+// CutHalfSin (rsq_load.cpp:393-410): one raw sample per output sample;
+// grid (column tiles, output rows, sinograms)
 template <typename T>
-__global__ void __launch_bounds__(256) rsq_kernel(RsqArgs a)
+__global__ void __launch_bounds__(256) rsq_cut_kernel(RsqArgs a)
 {
-    const size_t per = (size_t)a.sy * a.sx;
-    const size_t total = per * a.n;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
-         i += (size_t)gridDim.x * blockDim.x) {
-        const int k = (int)(i / per);
-        const size_t rem = i - (size_t)k * per;
-        const int r = (int)(rem / a.sx), c = (int)(rem - (size_t)r * a.sx);
-        const uint16_t *blk = a.raw + (size_t)k * (a.sin_y + 2) * a.sin_x;
-        double v;
-        if (!a.complete) {
-            // CutHalfSin (rsq_load.cpp:393-410)
-            v = line_integral(a, blk, a.flip ? a.sin_y - 1 - r : r, a.axis_move + c);
-        } else {
-            // Half2FullFan (rsq_load.cpp:327-391): output row r is angle f
-            const int f = a.flip ? a.sy - 1 - r + a.start : r;
-            const int c0 = a.sin_x - 1 - a.axis_move;   // column of the central ray
-            if (c >= c0) {
-                v = line_integral(a, blk, f, a.axis_move + c - c0);
-            } else {
-                const int t = c - c0;                    // < 0
-                const double at = a.at[-t];              // atan(t / distance), host table
-                const double fi = f * a.delta_fi;
-                double nf = (fi - 2 * at - kPi) / a.delta_fi;
-                if (nf > a.sin_y - 1 || nf < 0) nf = (fi - 2 * at + kPi) / a.delta_fi;
-                const double ca = ceil(nf), fb = floor(nf);
-                int ia = (int)ca, ib = (int)fb;
-                // inside the scan for delta_fi = 2 pi / sin_y (ia <= sin_y wraps
-                // to row 0 as in the reference); any other angle wraps too
-                ia = ((ia % a.sin_y) + a.sin_y) % a.sin_y;
-                ib = ((ib % a.sin_y) + a.sin_y) % a.sin_y;
-                const int col = -t + a.axis_move;
-                const double A = line_integral(a, blk, ia, col);
-                if (ca != fb) {
-                    const double B = line_integral(a, blk, ib, col);
-                    v = A + (B - A) * (-nf + ca);
-                } else {
-                    v = A;
-                }
-            }
+    const int c = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y, k = blockIdx.z;
+    if (c >= a.sx) return;
+    const uint16_t *blk = a.raw + (size_t)k * (a.sin_y + 2) * a.sin_x;
+    const double v = line_integral(a, blk, a.flip ? a.sin_y - 1 - r : r, a.axis_move + c);
+    static_cast<T *>(a.out)[((size_t)k * a.sy + r) * a.sx + c] = (T)v;
+}
+
+// First pass of Half2FullFan (rsq_load.cpp:327-391): the line integrals of
+// columns axis_move .. sin_x-1 (the only ones the rebinning reads) of
+// sinograms k0 .. k0+gridDim.z-1 into lines [gridDim.z][sin_y][sin_x -
+// axis_move], and the measured half of the output (columns from the central
+// ray c0 on are these values of angle row f, written to output row r).
+// Grid (column tiles, rows, sinograms).
+template <typename T>
+__global__ void __launch_bounds__(256) rsq_lines_kernel(RsqArgs a, int k0, double *lines)
+{
+    const int w = a.sin_x - a.axis_move, c0 = a.sin_x - 1 - a.axis_move;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, f = blockIdx.y, k = blockIdx.z;
+    if (x >= w) return;
+    const uint16_t *blk = a.raw + (size_t)(k0 + k) * (a.sin_y + 2) * a.sin_x;
+    const double v = line_integral(a, blk, f, a.axis_move + x);
+    lines[((size_t)k * a.sin_y + f) * w + x] = v;
+    const int r = a.flip ? a.sy - 1 - f + a.start : f;
+    if (r >= 0 && r < a.sy)
+        static_cast<T *>(a.out)[((size_t)(k0 + k) * a.sy + r) * a.sx + c0 + x] = (T)v;
+}
+
+// Second pass of Half2FullFan: the columns left of the central ray, the
+// redundant rays, interpolated between two angle rows of the line integrals.
+// The angle rows and the weight depend on (r, c) only, so each thread finds
+// them once and applies them to the nk sinograms of the group.  Grid (column
+// tiles over 0 .. c0-1, output rows).
+template <typename T>
+__global__ void __launch_bounds__(256) rsq_rebin_kernel(RsqArgs a, int k0, int nk,
+                                                        const double *lines)
+{
+    const int w = a.sin_x - a.axis_move, c0 = a.sin_x - 1 - a.axis_move;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y;
+    if (c >= c0) return;
+    const size_t lstride = (size_t)a.sin_y * w, ostride = (size_t)a.sy * a.sx;
+    T *out = static_cast<T *>(a.out) + (size_t)k0 * ostride + (size_t)r * a.sx + c;
+    const int f = a.flip ? a.sy - 1 - r + a.start : r;   // angle row of output row r
+    const int t = c - c0;                        // < 0
+    const double at = a.at[-t];                  // atan(t / distance), host table
+    const double fi = f * a.delta_fi;
+    double nf = (fi - 2 * at - kPi) / a.delta_fi;
+    if (nf > a.sin_y - 1 || nf < 0) nf = (fi - 2 * at + kPi) / a.delta_fi;
+    const double ca = ceil(nf), fb = floor(nf);
+    int ia = (int)ca, ib = (int)fb;
+    // inside the scan for delta_fi = 2 pi / sin_y (ia <= sin_y wraps to row 0
+    // as in the reference); any other angle wraps too
+    if ((unsigned)ia >= (unsigned)a.sin_y) ia = ((ia % a.sin_y) + a.sin_y) % a.sin_y;
+    if ((unsigned)ib >= (unsigned)a.sin_y) ib = ((ib % a.sin_y) + a.sin_y) % a.sin_y;
+    const double *LA = lines + (size_t)ia * w + (-t);   // column axis_move - t
+    const double *LB = lines + (size_t)ib * w + (-t);
+    const double wgt = -nf + ca;
+    const bool two = ca != fb;
+    for (int k = 0; k < nk; ++k) {
+        const double A = LA[k * lstride];
+        double v = A;
+        if (two) {
+            const double B = LB[k * lstride];
+            v = A + (B - A) * wgt;
         }
-        static_cast<T *>(a.out)[i] = (T)v;
+        out[k * ostride] = (T)v;
     }
 }
 
@@ -277,7 +312,6 @@ extern "C" int fdk_rsq_process(const fdk_rsq_params *p, double bh_a, double bh_b
     RsqArgs a;
     a.raw = d_raw;
     a.out = d_out;
-    a.n = nsin;
     a.sin_x = p->sin_x;
     a.sin_y = p->sin_y;
     out_size(p, &a.sx, &a.sy);
@@ -290,27 +324,58 @@ extern "C" int fdk_rsq_process(const fdk_rsq_params *p, double bh_a, double bh_b
     a.B = bh_b;
     a.distance = p->distance;
     a.delta_fi = p->delta_fi;
-    const size_t total = (size_t)a.sx * a.sy * nsin;
-    if (total == 0) return FDK_OK;
-    const unsigned blocks = (unsigned)std::min<size_t>((total + 255) / 256, 65536);
+    if ((size_t)a.sx * a.sy * nsin == 0) return FDK_OK;
+    // grids are (column tiles, rows, sinograms)
+    if (a.sin_y > 65535)
+        return fail(FDK_ERR_INVALID_ARGUMENT, "rsq: more than 65535 projections per sinogram");
     hipStream_t st = (hipStream_t)stream;
+    const bool f64 = out_dtype == FDK_F64;
+    hipError_t e = hipSuccess;
+    if (!a.complete) {
+        a.at = nullptr;
+        for (int k0 = 0; k0 < nsin && e == hipSuccess; k0 += 65535) {
+            RsqArgs b = a;
+            const int nk = std::min(nsin - k0, 65535);
+            b.raw = a.raw + (size_t)k0 * (a.sin_y + 2) * a.sin_x;
+            b.out = (char *)a.out + (size_t)k0 * a.sy * a.sx * (f64 ? 8 : 4);
+            const dim3 grid((a.sx + 255) / 256, a.sy, nk);
+            if (f64) hipLaunchKernelGGL(rsq_cut_kernel<double>, grid, dim3(256), 0, st, b);
+            else hipLaunchKernelGGL(rsq_cut_kernel<float>, grid, dim3(256), 0, st, b);
+            e = hipGetLastError();
+        }
+        if (e != hipSuccess) return tkv::hip_fail(e, "rsq");
+        return FDK_OK;
+    }
     // atan table, index -t for t = -1 .. -(sin_x - 1 - axis_move) (entry 0 unused)
-    const int nt = p->sin_x - p->axis_move;
-    std::vector<double> at(nt, 0.0);
+    const int w = p->sin_x - p->axis_move;
+    std::vector<double> at(w, 0.0);
     for (int t = -p->sin_x + 1 + p->axis_move; t < 0; ++t) at[-t] = std::atan(t / p->distance);
-    double *d_at = nullptr;
-    TKV_HIP(hipMalloc((void **)&d_at, nt * sizeof(double)));
-    hipError_t e = hipMemcpyAsync(d_at, at.data(), nt * sizeof(double), hipMemcpyHostToDevice, st);
+    const size_t per = (size_t)a.sin_y * w * sizeof(double);
+    const int group = (int)std::max<size_t>(1, std::min<size_t>({(size_t)nsin, kGroupBytes / per,
+                                                                 (size_t)65535}));
+    double *d_at = nullptr, *d_lines = nullptr;
+    TKV_HIP(hipMalloc((void **)&d_at, w * sizeof(double)));
+    e = hipMalloc((void **)&d_lines, per * group);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d_at, at.data(), w * sizeof(double), hipMemcpyHostToDevice, st);
     a.at = d_at;
-    if (e == hipSuccess) {
-        if (out_dtype == FDK_F64)
-            hipLaunchKernelGGL(rsq_kernel<double>, dim3(blocks), dim3(256), 0, st, a);
-        else
-            hipLaunchKernelGGL(rsq_kernel<float>, dim3(blocks), dim3(256), 0, st, a);
+    const int c0 = p->sin_x - 1 - p->axis_move;
+    for (int k0 = 0; k0 < nsin && e == hipSuccess; k0 += group) {
+        const int nk = std::min(group, nsin - k0);
+        const dim3 g1((w + 255) / 256, a.sin_y, nk), g2((c0 + 255) / 256, a.sy);
+        if (f64) hipLaunchKernelGGL(rsq_lines_kernel<double>, g1, dim3(256), 0, st, a, k0, d_lines);
+        else hipLaunchKernelGGL(rsq_lines_kernel<float>, g1, dim3(256), 0, st, a, k0, d_lines);
+        if (c0 > 0) {
+            if (f64) hipLaunchKernelGGL(rsq_rebin_kernel<double>, g2, dim3(256), 0, st, a, k0,
+                                        nk, (const double *)d_lines);
+            else hipLaunchKernelGGL(rsq_rebin_kernel<float>, g2, dim3(256), 0, st, a, k0, nk,
+                                    (const double *)d_lines);
+        }
         e = hipGetLastError();
     }
-    // the table lives until the kernel is done
+    // the table and the scratch live until the kernels are done
     if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(d_lines);
     (void)hipFree(d_at);
     if (e != hipSuccess) return tkv::hip_fail(e, "rsq");
     return FDK_OK;
