@@ -58,3 +58,40 @@ def test_reconstruction_to_tiff(tmp_path):
     for k in (0, g.Nz // 2, g.Nz - 1):
         _, img = read_tiff(str(tmp_path / f"{k:08d}.tif"))
         assert np.array_equal(img, O.to_u16(vol[k], lo, hi))
+
+
+@pytest.mark.parametrize("n", [1, 6, 4 * 1000 + 3, 100003])
+def test_range_and_scaling_at_any_alignment(n):
+    """fdk_volume_minmax and fdk_volume_to_u16 read four floats at a time when
+    the pointers allow it and one at a time otherwise; every start offset and a
+    length with a remainder give the oracle's range and 16-bit values."""
+    L = _lib.load()
+    rng = np.random.default_rng(n)
+    vol = (rng.standard_normal(n + 8) * 3).astype(np.float32)
+    vol[::7] = np.nan
+    vol[1] = -0.0
+    dev = DeviceBuffer(vol.nbytes)
+    out = DeviceBuffer(2 * (n + 8))
+    dev.upload(vol)
+    try:
+        for off in (0, 1, 2, 3, 4):
+            x = vol[off:off + n]
+            say(f"tiff range/scaling n={n} float offset {off}")
+            mm = (C.c_double * 2)()
+            rc = L.fdk_volume_minmax(C.c_void_p(dev.ptr.value + 4 * off), n, mm, None)
+            if np.isnan(x).all():
+                assert rc == _lib.FDK_ERR_INVALID_ARGUMENT   # no value to scale by
+                continue
+            assert rc == 0
+            lo, hi = float(np.nanmin(x)), float(np.nanmax(x))
+            assert (mm[0], mm[1]) == (lo, hi), off
+            for oo in (0, 1):
+                _lib.check(L.fdk_volume_to_u16(C.c_void_p(dev.ptr.value + 4 * off), n, lo, hi,
+                                               C.c_void_p(out.ptr.value + 2 * oo), None))
+                hip().hipDeviceSynchronize()
+                got = np.empty(n + 8, np.uint16)
+                out.download(got)
+                assert np.array_equal(got[oo:oo + n], O.to_u16(x, lo, hi)), (off, oo)
+    finally:
+        dev.close()
+        out.close()

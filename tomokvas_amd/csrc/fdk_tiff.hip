@@ -58,18 +58,35 @@ __global__ void minmax_init_kernel(unsigned *keys)
     }
 }
 
+// VEC: v is 16-byte aligned; the body reads float4 and the last n % 4
+// values one by one
+template <bool VEC>
 __global__ void __launch_bounds__(256) minmax_kernel(const float *v, size_t n, unsigned *keys)
 {
     unsigned lo = 0xffffffffu, hi = 0u;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (size_t)gridDim.x * blockDim.x) {
-        const float x = v[i];
+    auto take = [&](float x) {
         if (x == x) {
             const unsigned k = order_key(x);
             lo = k < lo ? k : lo;
             hi = k > hi ? k : hi;
         }
+    };
+    const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t nth = (size_t)gridDim.x * blockDim.x;
+    size_t done = 0;
+    if (VEC) {
+        const float4 *v4 = reinterpret_cast<const float4 *>(v);
+        const size_t n4 = n / 4;
+        for (size_t i = tid; i < n4; i += nth) {
+            const float4 q = v4[i];
+            take(q.x);
+            take(q.y);
+            take(q.z);
+            take(q.w);
+        }
+        done = 4 * n4;
     }
+    for (size_t i = done + tid; i < n; i += nth) take(v[i]);
     for (int o = 32; o > 0; o >>= 1) {
         const unsigned l2 = __shfl_xor(lo, o), h2 = __shfl_xor(hi, o);
         lo = l2 < lo ? l2 : lo;
@@ -92,12 +109,28 @@ __global__ void __launch_bounds__(256) minmax_kernel(const float *v, size_t n, u
     }
 }
 
+// VEC: v is 16-byte and out 8-byte aligned; the body converts four values per
+// step, the last n % 4 one by one
+template <bool VEC>
 __global__ void __launch_bounds__(256) to_u16_kernel(const float *v, size_t n, double vmin,
                                                      double vmax, uint16_t *out)
 {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (size_t)gridDim.x * blockDim.x)
-        out[i] = scale_u16(v[i], vmin, vmax);
+    const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t nth = (size_t)gridDim.x * blockDim.x;
+    size_t done = 0;
+    if (VEC) {
+        const float4 *v4 = reinterpret_cast<const float4 *>(v);
+        uint2 *o4 = reinterpret_cast<uint2 *>(out);
+        const size_t n4 = n / 4;
+        for (size_t i = tid; i < n4; i += nth) {
+            const float4 q = v4[i];
+            const unsigned a = scale_u16(q.x, vmin, vmax), b = scale_u16(q.y, vmin, vmax);
+            const unsigned c = scale_u16(q.z, vmin, vmax), d = scale_u16(q.w, vmin, vmax);
+            o4[i] = make_uint2(a | (b << 16), c | (d << 16));
+        }
+        done = 4 * n4;
+    }
+    for (size_t i = done + tid; i < n; i += nth) out[i] = scale_u16(v[i], vmin, vmax);
 }
 
 unsigned grid_for(size_t n) { return (unsigned)std::min<size_t>((n + 255) / 256, 8192); }
@@ -200,7 +233,12 @@ extern "C" int fdk_volume_minmax(const float *d_vol, size_t n, double *minmax, v
     TKV_HIP(hipMalloc((void **)&keys, 2 * sizeof(unsigned)));
     unsigned hk[2] = {0, 0};
     hipLaunchKernelGGL(minmax_init_kernel, dim3(1), dim3(64), 0, st, keys);
-    hipLaunchKernelGGL(minmax_kernel, dim3(grid_for(n)), dim3(256), 0, st, d_vol, n, keys);
+    if (((uintptr_t)d_vol & 15) == 0)
+        hipLaunchKernelGGL(minmax_kernel<true>, dim3(grid_for(n)), dim3(256), 0, st, d_vol, n,
+                           keys);
+    else
+        hipLaunchKernelGGL(minmax_kernel<false>, dim3(grid_for(n)), dim3(256), 0, st, d_vol, n,
+                           keys);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(hk, keys, sizeof hk, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
@@ -218,8 +256,12 @@ extern "C" int fdk_volume_to_u16(const float *d_vol, size_t n, double vmin, doub
 {
     if (!d_vol || !d_out) return fail(FDK_ERR_INVALID_ARGUMENT, "to_u16: null buffer");
     if (n == 0) return FDK_OK;
-    hipLaunchKernelGGL(to_u16_kernel, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream,
-                       d_vol, n, vmin, vmax, d_out);
+    if (((uintptr_t)d_vol & 15) == 0 && ((uintptr_t)d_out & 7) == 0)
+        hipLaunchKernelGGL(to_u16_kernel<true>, dim3(grid_for(n)), dim3(256), 0,
+                           (hipStream_t)stream, d_vol, n, vmin, vmax, d_out);
+    else
+        hipLaunchKernelGGL(to_u16_kernel<false>, dim3(grid_for(n)), dim3(256), 0,
+                           (hipStream_t)stream, d_vol, n, vmin, vmax, d_out);
     TKV_HIP(hipGetLastError());
     return FDK_OK;
 }
