@@ -12,7 +12,9 @@
  * fdk_tiff_export_device works on a volume in HBM (after a resident
  * reconstruction, fdk_device_volume): min/max and the 16-bit conversion run on
  * the GPU, only 2 bytes per voxel cross to the host.  fdk_tiff_export_host
- * does the same from host memory. */
+ * does the same from host memory.  Both write the files on up to 8 threads;
+ * on a failure the first failing file's error is returned (files of other
+ * slices may have been written). */
 #ifndef TOMOKVAS_FDK_TIFF_H
 #define TOMOKVAS_FDK_TIFF_H
 

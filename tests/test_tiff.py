@@ -64,3 +64,24 @@ def test_only_sixteen_bit_grey(tmp_path):
         f.SetTiffType(BW8)
     with pytest.raises(_lib.FDKError):
         _lib.check(_lib.load().fdk_tiff_write_u16(str(tmp_path / "x.tif").encode(), None, 0, 0))
+
+
+def test_host_export_many_slices_and_failure(tmp_path):
+    """fdk_tiff_export_host writes its slices on several threads: every file is
+    there with the oracle's pixels, and a directory that does not exist fails
+    with a message naming a file in it."""
+    L = _lib.load()
+    rng = np.random.default_rng(7)
+    vol = rng.standard_normal((37, 5, 9)).astype(np.float32)
+    lo, hi = float(vol.min()), float(vol.max())
+    out = tmp_path / "many"
+    out.mkdir()
+    _lib.check(L.fdk_tiff_export_host(vol.ctypes.data, 9, 5, 37, lo, hi, str(out).encode(), 11))
+    assert sorted(os.listdir(out)) == [f"{k:08d}.tif" for k in range(11, 48)]
+    for k in (0, 17, 36):
+        _, img = read_tiff(str(out / f"{k + 11:08d}.tif"))
+        assert np.array_equal(img, O.to_u16(vol[k], lo, hi))
+    missing = tmp_path / "absent"
+    with pytest.raises(_lib.FDKError, match="absent"):
+        _lib.check(L.fdk_tiff_export_host(vol.ctypes.data, 9, 5, 37, lo, hi,
+                                          str(missing).encode(), 0))

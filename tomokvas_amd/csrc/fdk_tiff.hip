@@ -12,6 +12,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -165,6 +166,50 @@ int write_slices(const uint16_t *img, int x, int y, int count, const char *dir, 
     return FDK_OK;
 }
 
+// threads that write the files of one slab (files are independent; the
+// page-cache copy of one thread runs at a few GB/s)
+int writer_threads(int count)
+{
+    const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
+    return std::max(1, std::min({8, hw, count}));
+}
+
+// slices [0, count) of img over `threads` threads, slice k on thread k %
+// threads; produce(t, k) may fill the slice first (a no-op when img already
+// holds it).  Returns the lowest-numbered thread's failure with its message.
+template <typename Produce>
+int write_slices_parallel(int count, int threads, std::string &err, Produce produce)
+{
+    std::vector<int> rcs(threads, FDK_OK);
+    std::vector<std::string> msgs(threads);
+    auto part = [&](int t) {
+        for (int k = t; k < count; k += threads) {
+            const int rc = produce(t, k);
+            if (rc) {
+                rcs[t] = rc;
+                msgs[t] = fdk_last_error_string();
+                return;
+            }
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < threads; ++t) {
+        try {
+            pool.emplace_back(part, t);
+        } catch (const std::system_error &) {
+            part(t);   // no thread available: this one writes its share
+        }
+    }
+    part(0);
+    for (auto &th : pool) th.join();
+    for (int t = 0; t < threads; ++t)
+        if (rcs[t]) {
+            err = msgs[t];
+            return rcs[t];
+        }
+    return FDK_OK;
+}
+
 }  // namespace
 
 extern "C" int fdk_tiff_write_u16(const char *path, const uint16_t *img, int width, int height)
@@ -303,8 +348,9 @@ extern "C" int fdk_tiff_export_device(const float *d_vol, int x, int y, int z, c
         e = hipMemcpy(h, d_u16, (size_t)nz * plane * 2, hipMemcpyDeviceToHost);
         if (e != hipSuccess) break;
         auto write = [&, h, nz, z0]() {
-            wrc = write_slices(h, x, y, nz, dir, first_index + z0);
-            if (wrc) werr = fdk_last_error_string();
+            wrc = write_slices_parallel(nz, writer_threads(nz), werr, [&](int, int k) {
+                return write_slices(h + (size_t)k * plane, x, y, 1, dir, first_index + z0 + k);
+            });
         };
         try {
             writer = std::thread(write);
@@ -328,12 +374,15 @@ extern "C" int fdk_tiff_export_host(const float *vol, int x, int y, int z, doubl
     if (!vol || x < 1 || y < 1 || z < 1)
         return fail(FDK_ERR_INVALID_ARGUMENT, "tiff export: empty volume");
     const size_t plane = (size_t)x * y;
-    std::vector<uint16_t> img(plane);
-    for (int k = 0; k < z; ++k) {
+    const int threads = writer_threads(z);
+    std::vector<std::vector<uint16_t>> img(threads, std::vector<uint16_t>(plane));
+    std::string err;
+    const int rc = write_slices_parallel(z, threads, err, [&](int t, int k) {
         const float *s = vol + (size_t)k * plane;
-        for (size_t i = 0; i < plane; ++i) img[i] = scale_u16(s[i], vmin, vmax);
-        int rc = write_slices(img.data(), x, y, 1, dir, first_index + k);
-        if (rc) return rc;
-    }
+        uint16_t *d = img[t].data();
+        for (size_t i = 0; i < plane; ++i) d[i] = scale_u16(s[i], vmin, vmax);
+        return write_slices(d, x, y, 1, dir, first_index + k);
+    });
+    if (rc) return fail(rc, err);
     return FDK_OK;
 }
