@@ -39,6 +39,9 @@ MODES = [  # complete, minimal, flip, invert, axis_move
     (True, False, True, True, 7),
     (True, True, False, True, 3),
     (True, True, True, False, 3),
+    # axis on the last column: one output column, no redundant half
+    (True, False, True, False, SX - 1),
+    (False, False, False, True, SX - 1),
 ]
 
 
