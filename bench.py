@@ -226,38 +226,12 @@ def read_profile(workload: str, storage: str):
     return None, None
 
 
-def sub_cube(g, seed: int, n: int = 64):
-    """Seeded n^3 voxel block whose centre lies within 256 px of the isocentre."""
-    rng = np.random.default_rng(seed)
-    half = int(256 / g.scale)
-    cx, cy, cz = g.Nx // 2, g.Ny // 2, g.Nz // 2
-    x0 = int(np.clip(cx + rng.integers(-half, half) - n // 2, 0, g.Nx - n))
-    y0 = int(np.clip(cy + rng.integers(-half, half) - n // 2, 0, g.Ny - n))
-    z0 = int(np.clip(cz + rng.integers(-half, half) - n // 2, 0, g.Nz - n))
-    return x0, y0, z0, n
-
-
-def detector_rows_of_block(g, x0, y0, z0, n):
-    """Row range [v_lo, v_hi) containing every sample the block reads."""
-    ax = g.scale * (np.arange(x0, x0 + n) - g.cx)
-    ay = g.scale * (np.arange(y0, y0 + n) - g.cy)
-    az = g.scale * (np.array([z0, z0 + n - 1]) - (g.Nz - 1) / 2 - g.cz)
-    b = np.arange(g.Nang) * g.dbeta + g.rot
-    X, Y = np.meshgrid(ax, ay)
-    S = X.ravel()[:, None] * np.cos(b)[None, :] + Y.ravel()[:, None] * np.sin(b)[None, :]
-    U = g.D / (g.D - S)
-    v = np.concatenate([(az[0] * U).ravel(), (az[1] * U).ravel()]) + g.cZ
-    lo = max(0, int(math.floor(v.min())) - 2)
-    hi = min(g.Nv, int(math.ceil(v.max())) + 3)
-    return lo, hi
-
-
 def cpu_leg(fbp, w, g, block_vols, hipm, threads):
     """CPU baseline (FP32 mirror, timed) and FP64 oracle on a seeded 64^3 block."""
     import oracle as O
     os.environ.setdefault("OMP_NUM_THREADS", str(threads))
-    x0, y0, z0, n = sub_cube(g, w.seed)
-    lo, hi = detector_rows_of_block(g, x0, y0, z0, n)
+    x0, y0, z0, n = geometry.sub_cube(g, w.seed)
+    lo, hi = geometry.detector_rows_of_block(g, x0, y0, z0, n)
     rows = hi - lo
     log(f"cpu leg: block x{x0} y{y0} z{z0} n{n}, detector rows [{lo},{hi})")
     raw = np.empty((rows, g.Nang, g.Nu), np.float32)
@@ -360,7 +334,7 @@ def main():
         order.remove(_lib.F32)
         order.insert(0, _lib.F32)
     res = {}
-    block = sub_cube(g, w.seed)
+    block = geometry.sub_cube(g, w.seed)
     block_vols = {}
     for st in order:
         _lib.check(fbp.SetStorage(st), "SetStorage")

@@ -417,3 +417,110 @@ def test_engine_matches_golden_fixture(torch):
     stack, scales = st.pack(s, _lib.F64)
     v64 = st.backproject(stack, _lib.F64, scales).cpu().numpy()
     assert O.rel_rmse(v64, fx["fdk_vol64"]) <= 1e-12
+
+
+# ------------------------------------------------------------------ full size (cfg3)
+def test_cfg3_precision_form(torch):
+    """1024^3 from 1800 x 2048^2 (64 seeded ellipsoids + Poisson, Shepp-Logan
+    filter) through fdk_calculate_reconstruction with FP32, FP16 and BF16
+    storage.  A seeded 64^3 block is checked against the CPU FP32 mirror and
+    the FP64 oracle run on the block's detector rows only (weight and filter
+    act on rows independently); FP16/BF16 volumes against the FP32 volume."""
+    import ctypes as C
+    from tomokvas_amd import _hip
+    from tomokvas_amd.engine import FilteredBackProjection
+    w = geometry.workloads()["cfg3"]
+    g = w.geom
+    L = _lib.load()
+    say("cfg3 full size: 1024^3 from 1800 x 2048^2, FP32 / FP16 / BF16 storage")
+    fbp = FilteredBackProjection(0)
+    g.apply(fbp, None, None)
+    _lib.check(fbp.SetFilter(w.filter), "SetFilter")
+    _lib.check(fbp.SetStorage(_lib.F32), "SetStorage")
+    _lib.check(fbp.Initialize(), "Initialize")
+    ells = np.ascontiguousarray(w.ellipsoids())
+    _lib.check(L.fdk_phantom_project(fbp.handle, ells.ctypes.data, ells.shape[0], 0, g.Nang,
+                                     fbp.device_sinogram_ptr(), None), "phantom")
+    _lib.check(L.fdk_poisson_log(fbp.device_sinogram_ptr(), g.Nu * g.Nv * g.Nang, w.I0, w.seed,
+                                 None), "poisson")
+    hip = _hip.hip()
+    x0, y0, z0, n = geometry.sub_cube(g, w.seed)
+    lo, hi = geometry.detector_rows_of_block(g, x0, y0, z0, n)
+    raw = np.empty((hi - lo, g.Nang, g.Nu), np.float32)
+    assert hip.hipMemcpy(raw.ctypes.data, C.c_void_p(fbp.device_sinogram_ptr()
+                                                     + lo * g.Nang * g.Nu * 4),
+                         raw.nbytes, 2) == 0
+    nvol = g.Nx * g.Ny * g.Nz
+    ref = torch.empty(nvol, dtype=torch.float32, device="cuda")
+    blocks = {}
+    for st, name in ((_lib.F32, "f32"), (_lib.F16, "f16"), (_lib.BF16, "bf16")):
+        _lib.check(fbp.SetStorage(st), "SetStorage")
+        _lib.check(L.fdk_calculate_reconstruction(fbp.handle, _lib.GET_BLOCK_FN(), None), name)
+        t = fbp.stage_times_ms()
+        dvol = fbp.device_volume_ptr()
+        blk = np.empty((n, g.Ny, g.Nx), np.float32)
+        assert hip.hipMemcpy(blk.ctypes.data, C.c_void_p(dvol + z0 * g.Nx * g.Ny * 4),
+                             blk.nbytes, 2) == 0
+        blocks[name] = blk[:, y0:y0 + n, x0:x0 + n].reshape(-1)
+        if st == _lib.F32:
+            assert hip.hipMemcpy(C.c_void_p(ref.data_ptr()), C.c_void_p(dvol), nvol * 4, 3) == 0
+            say(f"  {name}: filter {t[0]:.1f} ms, pack {t[1]:.1f} ms, BP {t[2]:.1f} ms")
+        else:
+            out3 = (C.c_double * 3)()
+            _lib.check(L.fdk_compare(C.c_void_p(dvol), C.c_void_p(ref.data_ptr()), nvol, out3,
+                                     None), "compare")
+            e = math.sqrt(out3[0] / out3[1])
+            say(f"  {name}: filter {t[0]:.1f} ms, pack {t[1]:.1f} ms, BP {t[2]:.1f} ms; "
+                f"full volume vs FP32 rel RMSE {e:.3e}")
+            assert e <= TOL_F16, name
+    fbp.close()
+    # the block on the CPU: FP32 mirror and FP64 oracle from the same raw rows
+    gd = g.oracle_dict()
+    gd.update(Nv=hi - lo, cZ=g.cZ - lo)
+    og = O.Geometry(**gd)
+    zz, yy, xx = np.meshgrid(np.arange(z0, z0 + n), np.arange(y0, y0 + n),
+                             np.arange(x0, x0 + n), indexing="ij")
+    pts = np.stack([xx.ravel(), yy.ravel(), zz.ravel()], axis=1).astype(np.int32)
+    Hf = O.filter_spectrum(w.filter, g.Nu, g.tau)
+    m32 = O.backproject_points(og, O.filter_rows(og, O.weight(og, raw), Hf), pts)
+    raw64 = raw.astype(np.float64)
+    del raw
+    o64 = O.backproject_points(og, O.filter_rows(og, O.weight(og, raw64, f64=True), Hf, f64=True),
+                               pts, f64=True)
+    e_mirror = O.rel_rmse(m32, o64)
+    e = {k: O.rel_rmse(v, o64) for k, v in blocks.items()}
+    e_f32_mirror = O.rel_rmse(blocks["f32"], m32)
+    say(f"  block x{x0} y{y0} z{z0} n{n} (rows [{lo},{hi})): FP32 vs mirror {e_f32_mirror:.3e}; "
+        f"vs FP64 oracle: mirror {e_mirror:.3e}, " +
+        ", ".join(f"{k} {v:.3e}" for k, v in e.items()))
+    assert e_f32_mirror <= TOL_FDK
+    # FP32 rounding of the log-transformed input, amplified by the ramp filter,
+    # sets both FP32 paths' distance to FP64 (the mirror's included)
+    assert e["f32"] <= max(TOL_FDK, 2 * e_mirror)
+    assert e["f16"] <= TOL_F16 and e["bf16"] <= TOL_F16
+
+
+def test_set_gpu_id_moves_every_stream(cfg1):
+    # fdk_set_gpu_id makes every stream and event again on the (new) device;
+    # the slab pipeline (copy stream) and fdk_reconstruct_host (copy and prep
+    # streams) after it must give the results of a context that never moved
+    g, sino = cfg1
+    s32 = sino.astype(np.float32)
+    out = []
+    for move in (False, True):
+        fbp = engine(g, None, None)
+        if move:
+            _lib.check(fbp.SetGPUid(0), "SetGPUid")
+        sink = VolumeSink(g.Nx, g.Ny, g.Nz, g.B)
+        g.apply(fbp, s32, sink)
+        _lib.check(fbp.Initialize(), "Initialize")
+        _lib.check(fbp.CalculateReconstruction(), "CalculateReconstruction")
+        a = sink.volume.copy()
+        sink2 = VolumeSink(g.Nx, g.Ny, g.Nz, g.B)
+        g.apply(fbp, None, sink2)
+        _lib.check(fbp.ReconstructHost(s32), "ReconstructHost")
+        out.append((a, sink2.volume.copy()))
+        fbp.close()
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+    assert O.rel_rmse(out[1][1], out[1][0]) <= 1e-5

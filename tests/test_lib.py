@@ -98,3 +98,16 @@ def test_l2_probe_rejects_bad_footprints():
     for fp in (0, 1000, 3 << 20, 128 << 20):
         assert lib.fdk_probe_l2_read(fp, C.byref(gbs)) == _lib.FDK_ERR_INVALID_ARGUMENT
     assert lib.fdk_probe_l2_read(2 << 20, None) == _lib.FDK_ERR_INVALID_ARGUMENT
+
+
+def test_para_tc_host_program_builds_against_the_cpp_class(tmp_path):
+    # instantiates SetReconstructionProperties<Sink> and the GetBlock trampoline
+    # of the drop-in class and links against the library (the GPU test runs it)
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp"))
+    import build_host
+    exe = build_host.build(out=str(tmp_path / "para_tc_host"))
+    assert os.path.getsize(exe) > 0
+    env = dict(os.environ, LD_LIBRARY_PATH=_lib.HERE)   # the rpath is relative to tests/cpp
+    r = subprocess.run([exe], capture_output=True, text=True, env=env)
+    assert r.returncode == 2 and "usage" in r.stderr

@@ -506,6 +506,48 @@ int backproject_and_hand_out(fdk_ctx *c, fdk_get_block_fn get_block, void *user,
     return FDK_OK;
 }
 
+// The context's streams and events, all on device c->dev.  A failure leaves
+// every handle either valid or null, so destroy_streams may always follow.
+int create_streams(fdk_ctx *c)
+{
+    TKV_HIP(hipSetDevice(c->dev));
+    TKV_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    TKV_HIP(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
+    TKV_HIP(hipStreamCreateWithFlags(&c->prep_stream, hipStreamNonBlocking));
+    for (auto &e : c->ev) TKV_HIP(hipEventCreate(&e));
+    for (int i = 0; i < 2; ++i) {
+        TKV_HIP(hipEventCreateWithFlags(&c->ev_bp[i], hipEventDisableTiming));
+        TKV_HIP(hipEventCreateWithFlags(&c->ev_cp[i], hipEventDisableTiming));
+        TKV_HIP(hipEventCreateWithFlags(&c->ev_up[i], hipEventDisableTiming));
+        TKV_HIP(hipEventCreateWithFlags(&c->ev_f[i], hipEventDisableTiming));
+    }
+    return FDK_OK;
+}
+
+// destroys every stream and event of c (the current device must be c->dev)
+// and nulls the handles
+void destroy_streams(fdk_ctx *c)
+{
+    auto drop_event = [](hipEvent_t &e) {
+        if (e) (void)hipEventDestroy(e);
+        e = nullptr;
+    };
+    auto drop_stream = [](hipStream_t &s) {
+        if (s) (void)hipStreamDestroy(s);
+        s = nullptr;
+    };
+    for (auto &e : c->ev) drop_event(e);
+    for (int i = 0; i < 2; ++i) {
+        drop_event(c->ev_bp[i]);
+        drop_event(c->ev_cp[i]);
+        drop_event(c->ev_up[i]);
+        drop_event(c->ev_f[i]);
+    }
+    drop_stream(c->stream);
+    drop_stream(c->copy_stream);
+    drop_stream(c->prep_stream);
+}
+
 }  // namespace
 
 // ====================================================================== C ABI
@@ -525,20 +567,10 @@ int fdk_create(int gpu_id, fdk_ctx **out)
                                                   " devices)");
     fdk_ctx *c = new fdk_ctx();
     c->dev = gpu_id;
-    hipError_t e = hipSetDevice(gpu_id);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->prep_stream, hipStreamNonBlocking);
-    for (int i = 0; i < 5 && e == hipSuccess; ++i) e = hipEventCreate(&c->ev[i]);
-    for (int i = 0; i < 2 && e == hipSuccess; ++i) {
-        e = hipEventCreateWithFlags(&c->ev_bp[i], hipEventDisableTiming);
-        if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_cp[i], hipEventDisableTiming);
-        if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_up[i], hipEventDisableTiming);
-        if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_f[i], hipEventDisableTiming);
-    }
-    if (e != hipSuccess) {
+    const int rc = create_streams(c);
+    if (rc) {
         fdk_destroy(c);
-        return tkv::hip_fail(e, "fdk_create");
+        return rc;
     }
     *out = c;
     return FDK_OK;
@@ -553,17 +585,7 @@ int fdk_destroy(fdk_ctx *c)
     if (c->prep_stream) (void)hipStreamSynchronize(c->prep_stream);
     free_engine(c);
     free_tables(c);
-    for (auto &e : c->ev)
-        if (e) (void)hipEventDestroy(e);
-    for (int i = 0; i < 2; ++i) {
-        if (c->ev_bp[i]) (void)hipEventDestroy(c->ev_bp[i]);
-        if (c->ev_cp[i]) (void)hipEventDestroy(c->ev_cp[i]);
-        if (c->ev_up[i]) (void)hipEventDestroy(c->ev_up[i]);
-        if (c->ev_f[i]) (void)hipEventDestroy(c->ev_f[i]);
-    }
-    if (c->stream) (void)hipStreamDestroy(c->stream);
-    if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
-    if (c->prep_stream) (void)hipStreamDestroy(c->prep_stream);
+    destroy_streams(c);
     delete c;
     return FDK_OK;
 }
@@ -587,16 +609,14 @@ int fdk_set_gpu_id(fdk_ctx *c, int gpu_id)
     int n = 0;
     TKV_HIP(hipGetDeviceCount(&n));
     if (gpu_id < 0 || gpu_id >= n) return fail(FDK_ERR_INVALID_ARGUMENT, "GPU id out of range");
-    // the stream and events belong to the creation device: recreate them
+    // every stream and event belongs to the device it was created on: all of
+    // them are made again on the new device (also for the same id, so that one
+    // path serves every call; SetGPUid only records the id, kernel.cu:283-286,
+    // and the device is selected when the engine runs)
     (void)hipSetDevice(c->dev);
-    for (auto &e : c->ev)
-        if (e) (void)hipEventDestroy(e);
-    if (c->stream) (void)hipStreamDestroy(c->stream);
+    destroy_streams(c);
     c->dev = gpu_id;
-    TKV_HIP(hipSetDevice(gpu_id));
-    TKV_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-    for (auto &e : c->ev) TKV_HIP(hipEventCreate(&e));
-    return FDK_OK;
+    return create_streams(c);
 }
 
 int fdk_set_block_size(fdk_ctx *c, int slices)

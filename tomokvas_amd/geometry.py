@@ -134,3 +134,34 @@ def workloads() -> dict:
                          flat_detector(1536, 1536, 1200, 0.1, 500, 1000, 1024, 1024, 1024, 1.5),
                          "ellipsoids", 0)
     return w
+
+
+def sub_cube(g: ScanGeometry, seed: int, n: int = 64):
+    """Seeded n^3 voxel block (x0, y0, z0, n) whose centre lies within 256
+    virtual pixels of the isocentre: the voxel subsample that large volumes
+    are checked on against the FP64 oracle."""
+    rng = np.random.default_rng(seed)
+    half = int(256 / g.scale)
+    cx, cy, cz = g.Nx // 2, g.Ny // 2, g.Nz // 2
+    x0 = int(np.clip(cx + rng.integers(-half, half) - n // 2, 0, g.Nx - n))
+    y0 = int(np.clip(cy + rng.integers(-half, half) - n // 2, 0, g.Ny - n))
+    z0 = int(np.clip(cz + rng.integers(-half, half) - n // 2, 0, g.Nz - n))
+    return x0, y0, z0, n
+
+
+def detector_rows_of_block(g: ScanGeometry, x0: int, y0: int, z0: int, n: int):
+    """Detector row range [v_lo, v_hi) holding every sample the backprojection
+    of the block x0..x0+n-1, y0.., z0.. reads, over all angles (two rows of
+    margin each side); the cosine weight and the row filter act on rows
+    independently, so these rows alone reproduce the block."""
+    ax = g.scale * (np.arange(x0, x0 + n) - g.cx)
+    ay = g.scale * (np.arange(y0, y0 + n) - g.cy)
+    az = g.scale * (np.array([z0, z0 + n - 1]) - (g.Nz - 1) / 2 - g.cz)
+    b = np.arange(g.Nang) * g.dbeta + g.rot
+    X, Y = np.meshgrid(ax, ay)
+    S = X.ravel()[:, None] * np.cos(b)[None, :] + Y.ravel()[:, None] * np.sin(b)[None, :]
+    U = g.D / (g.D - S)
+    v = np.concatenate([(az[0] * U).ravel(), (az[1] * U).ravel()]) + g.cZ
+    lo = max(0, int(math.floor(v.min())) - 2)
+    hi = min(g.Nv, int(math.ceil(v.max())) + 3)
+    return lo, hi
