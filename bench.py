@@ -35,8 +35,9 @@ HBM_PEAK_GBS = 8000.0
 # non-packed VALU issue: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz (one wave64 instruction
 # per SIMD every 4 clocks)
 VALU_LANE_OPS = 256 * 4 * 16 * 2.4e9
-NAMES = {_lib.F32: "f32", _lib.F16: "f16", _lib.BF16: "bf16"}
-CELL_BYTES = {_lib.F32: 4, _lib.F16: 2, _lib.BF16: 2}   # bytes per stored sample
+NAMES = {_lib.F32: "f32", _lib.F16: "f16", _lib.BF16: "bf16", _lib.F16_PACKED: "f16-packed"}
+CODES = {v: k for k, v in NAMES.items()}
+CELL_BYTES = {_lib.F32: 4, _lib.F16: 2, _lib.BF16: 2, _lib.F16_PACKED: 2}   # bytes per stored sample
 
 
 def log(*a):
@@ -280,7 +281,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="cfg3")
-    ap.add_argument("--storages", default="f32,f16,bf16")
+    ap.add_argument("--storages", default="f32,f16,bf16,f16-packed")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--rsq", action="store_true",
                     help="also time the RSQ preprocessing (248 sinograms of 2000 x 1024)")
@@ -328,8 +329,7 @@ def main():
     ref = C.c_void_p()
     assert hipm.hipMalloc(C.byref(ref), nvol * 4) == 0
     nullcb = _lib.GET_BLOCK_FN()
-    order = [{"f32": _lib.F32, "f16": _lib.F16, "bf16": _lib.BF16}[s]
-             for s in args.storages.split(",")]
+    order = [CODES[s] for s in args.storages.split(",")]
     if _lib.F32 in order:
         order.remove(_lib.F32)
         order.insert(0, _lib.F32)
@@ -388,7 +388,7 @@ def main():
     head = "f16" if "f16" in res else NAMES[order[0]]
     r = res[head]
     upd = g.voxel_updates
-    s_bytes = CELL_BYTES[{"f32": _lib.F32, "f16": _lib.F16, "bf16": _lib.BF16}[head]]
+    s_bytes = CELL_BYTES[CODES[head]]
     gather = 4 * s_bytes * upd                      # 2x2 taps per voxel-update
     achieved = gather / (r["bp_ms"] * 1e-3) / 1e9   # GB/s
     traffic, valu_per_update = read_profile(args.workload, head)

@@ -37,9 +37,13 @@ extern "C" {
 
 /* ---- data types of stage-level buffers --------------------------------------------- */
 #define FDK_F32   0
-#define FDK_F16   1   /* filtered projections stored as FP16 with a per-projection scale */
+#define FDK_F16   1   /* filtered projections stored as FP16 with a per-projection scale;
+                         interpolation weights, lerps and accumulation in FP32 */
 #define FDK_BF16  2
 #define FDK_F64   3
+#define FDK_F16_PACKED 4 /* storage: FP16 as FDK_F16, but interpolated in packed FP16
+                            (FP16 fractions and column weights, v_dot2 into FP32 sums):
+                            faster, outside the FP32-interpolation precision form; opt-in */
 
 /* ---- ramp filters ------------------------------------------------------------------ */
 #define FDK_FILTER_RAMLAK      0   /* kernel.cu:783-802 (the reference's only filter)  */

@@ -24,8 +24,13 @@ def fma(a, b, c):  # float32 fma via float64 (exact product)
     return (np.float64(a) * np.float64(b) + np.float64(c)).astype(f32)
 
 BOX_W, ALIGN = 64, 4
-# rows of the LDS box per z-run length (box_rows in the kernel)
-BOX_ROWS = {16: 64, 24: 80, 32: 96, 48: 128, 64: 128}
+# the footprint's row limit (bh < limit, else the HBM gather) per kernel
+# configuration (z-run, limit): the backprojector's FP32-pair box of 64 rows
+# (24-slice runs, limit 65: bh <= 64 entry rows), its packed-FP16 box
+# (box_rows<32>() = 96), the adjoint's 64-slice box (box_rows<64>() = 128), and
+# the other box_rows<ZR>() sizes
+KERNELS = {"bp_pairs64_zr24": (24, 65), "bp_f16packed_zr32": (32, 96), "fp_zr64": (64, 128),
+           "zr16": (16, 64), "zr48": (48, 128)}
 
 
 def footprint(c, s, x0, x1, y0, y1, azlo, azhi, cX, cZ, D, Nu, Nv, box_h):
@@ -52,8 +57,7 @@ def footprint(c, s, x0, x1, y0, y1, azlo, azhi, cX, cZ, D, Nu, Nv, box_h):
     inner = bool(vlo >= f32(0.02) and vhi <= f32(Nv - 1) - f32(0.02))
     return (bu0, bv0, bw | (0x10000 if inner else 0), bh)
 
-def check(g, ZR=32, tile_stride=1, ang_stride=1):
-    box_h = BOX_ROWS[ZR]
+def check(g, ZR=32, tile_stride=1, ang_stride=1, box_h=96):
     """Counts of staged / empty / HBM-gathered (block, angle) pairs and taps."""
     ax, ay, az = [a.astype(f32) for a in O.axis_tables(g)]
     nz_pad = ((g.Nz + 63) // 64) * 64 + 64
@@ -133,11 +137,12 @@ CASES = {
 }
 
 
-@pytest.mark.parametrize("zr", sorted(BOX_ROWS))
+@pytest.mark.parametrize("kernel", list(KERNELS))
 @pytest.mark.parametrize("name", list(CASES))
-def test_footprint_boxes_cover_every_tap(name, zr):
+def test_footprint_boxes_cover_every_tap(name, kernel):
     make, ts, an = CASES[name]
-    st = check(make(), ZR=zr, tile_stride=ts, ang_stride=an)
+    zr, box_h = KERNELS[kernel]
+    st = check(make(), ZR=zr, tile_stride=ts, ang_stride=an, box_h=box_h)
     assert st["taps"] > 0
     if name.startswith("cfg") and (zr < 64 or name != "cfg3"):
         # the workloads never fall back to HBM gathers (64-slice runs are the

@@ -51,7 +51,8 @@ def test_engine_fdk_cfg1_matches_mirror(cfg1):
     assert sink.calls == [(n, False) for n in range(O.slab_count(g.Nz, g.B))]
 
 
-@pytest.mark.parametrize("storage,name", [(_lib.F16, "fp16"), (_lib.BF16, "bf16")])
+@pytest.mark.parametrize("storage,name", [(_lib.F16, "fp16"), (_lib.BF16, "bf16"),
+                                          (_lib.F16_PACKED, "fp16-packed")])
 def test_engine_low_precision_storage_cfg1(cfg1, storage, name):
     g, sino = cfg1
     vol, _, _ = run_engine(g, sino.astype(np.float32), storage=storage)
@@ -266,15 +267,51 @@ def test_fp16_scale_and_bf16_storage(torch, geo):
     d = torch.from_numpy(q).cuda()
     maxabs = torch.from_numpy(np.abs(q).max(axis=(0, 2)).astype(np.float32).view(np.int32)).cuda()
     ref = O.backproject(ogeom(g), q.astype(np.float64), f64=True)
-    for dt, tol in ((_lib.F16, 1e-3), (_lib.BF16, 4e-3)):
+    for dt, tol in ((_lib.F16, 1e-3), (_lib.BF16, 4e-3), (_lib.F16_PACKED, 1e-3)):
         stack, scales = st.pack(d, dt, maxabs)
-        if dt == _lib.F16:
+        if dt in (_lib.F16, _lib.F16_PACKED):
             s = scales.cpu().numpy()
             assert np.allclose(s, np.abs(q).max(axis=(0, 2)) / 8192.0, rtol=1e-7)
         v = st.backproject(stack, dt, scales).cpu().numpy()
         e = O.rel_rmse(v, ref)
         say(f"  storage {dt}: rel RMSE {e:.3e}")
         assert e <= tol
+
+
+@pytest.mark.parametrize("geo", ["odd", "cfg1", "wide"])
+def test_low_precision_storage_interpolates_in_fp32(torch, geo):
+    # FP16 and BF16 storage: the backprojector widens the stored samples to
+    # FP32 and interpolates, weights and sums in FP32, so its volume is the
+    # FP32 mirror's backprojection of the decoded stored samples up to FP32
+    # rounding (the FP16 scale multiplies the weight instead of each sample);
+    # the packed-FP16 kernel, with FP16 fractions and weights, is not
+    g = {"odd": lambda: scan(**ODD), "wide": lambda: scan(**WIDE),
+         "cfg1": lambda: geometry.workloads()["cfg1"].geom}[geo]()
+    st = stages(g)
+    rng = np.random.default_rng(9)
+    q = (rng.standard_normal((g.Nv, g.Nang, g.Nu)) * 100).astype(np.float32)
+    d = torch.from_numpy(q).cuda()
+    m = np.abs(q).max(axis=(0, 2)).astype(np.float32)
+    maxabs = torch.from_numpy(m.view(np.int32)).cuda()
+    pitch = (g.Nu + 68 + 63) // 64 * 64
+    n = g.Nang * g.Nv * pitch
+    for dt, view in ((_lib.F16, np.float16), (_lib.BF16, np.uint16)):
+        stack, scales = st.pack(d, dt, maxabs)
+        raw = stack.cpu().numpy().view(view)[:n].reshape(g.Nang, g.Nv, pitch)[:, :, :g.Nu]
+        if dt == _lib.F16:
+            dec = raw.astype(np.float32) * scales.cpu().numpy()[:, None, None]
+        else:
+            dec = (raw.astype(np.uint32) << 16).view(np.float32)
+        ref = O.backproject(ogeom(g), np.ascontiguousarray(dec.transpose(1, 0, 2)))
+        v = st.backproject(stack, dt, scales).cpu().numpy()
+        e = O.rel_rmse(v, ref)
+        say(f"  {geo} storage {dt}: vs FP32 mirror of the stored samples {e:.3e}")
+        assert e <= 1e-6
+        if dt == _lib.F16:
+            vp = st.backproject(stack, _lib.F16_PACKED, scales).cpu().numpy()
+            ep = O.rel_rmse(vp, ref)
+            say(f"  {geo} packed FP16 interpolation: {ep:.3e}")
+            assert 1e-5 < ep <= 1e-3
 
 
 @pytest.mark.parametrize("geo", ["odd", "wide"])
@@ -453,7 +490,8 @@ def test_cfg3_precision_form(torch):
     nvol = g.Nx * g.Ny * g.Nz
     ref = torch.empty(nvol, dtype=torch.float32, device="cuda")
     blocks = {}
-    for st, name in ((_lib.F32, "f32"), (_lib.F16, "f16"), (_lib.BF16, "bf16")):
+    for st, name in ((_lib.F32, "f32"), (_lib.F16, "f16"), (_lib.BF16, "bf16"),
+                     (_lib.F16_PACKED, "f16-packed")):
         _lib.check(fbp.SetStorage(st), "SetStorage")
         _lib.check(L.fdk_calculate_reconstruction(fbp.handle, _lib.GET_BLOCK_FN(), None), name)
         t = fbp.stage_times_ms()
@@ -497,7 +535,7 @@ def test_cfg3_precision_form(torch):
     # FP32 rounding of the log-transformed input, amplified by the ramp filter,
     # sets both FP32 paths' distance to FP64 (the mirror's included)
     assert e["f32"] <= max(TOL_FDK, 2 * e_mirror)
-    assert e["f16"] <= TOL_F16 and e["bf16"] <= TOL_F16
+    assert e["f16"] <= TOL_F16 and e["bf16"] <= TOL_F16 and e["f16-packed"] <= TOL_F16
 
 
 def test_set_gpu_id_moves_every_stream(cfg1):

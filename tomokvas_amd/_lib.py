@@ -25,6 +25,7 @@ FDK_ERR_CALLBACK = 5
 FDK_ERR_UNSUPPORTED = 6
 
 F32, F16, BF16, F64 = 0, 1, 2, 3
+F16_PACKED = 4   # FP16 storage interpolated in packed FP16 (opt-in, FP16 weights)
 FILTER_RAMLAK, FILTER_SHEPPLOGAN, FILTER_USER = 0, 1, 2
 VOLUME_RESIDENT, VOLUME_STREAMED = 0, 1
 

@@ -13,12 +13,20 @@
 //  * per angle, the block's detector footprint -- a box of samples that the
 //    workgroup derives from its corner columns (u and U are linear-fractional
 //    in (aX, aY), so their extremes over the block sit at the corners) -- is
-//    staged from HBM into LDS with coalesced loads and converted to FP32;
-//    every bilinear gather of the angle then reads LDS, so L2/HBM carry about
-//    one stored sample per two voxel-updates instead of four taps per update;
+//    staged from HBM into LDS with coalesced loads; every bilinear gather of
+//    the angle then reads LDS, so L2/HBM carry about one stored sample per two
+//    voxel-updates instead of four taps per update;
 //  * the stored stack is plain and angle-major ([beta][v][pitch]) in the
 //    storage type: FP32, FP16 with a per-projection scale, or BF16; FP16 and
-//    BF16 halve the staging bytes; interpolation and accumulation are FP32;
+//    BF16 halve the staging bytes.  FP32 and BF16 boxes hold FP32 samples.
+//    FP16 boxes keep FP16 sample pairs (q[c], q[c+1]) per 32-bit word, which
+//    halves the LDS bytes of a gather; the FP16 path (FDK_F16) widens them
+//    inside mixed-precision FMAs (v_fma_mix_f32: FP16 samples, FP32 fraction,
+//    differences and weights, FP32 result), so for every storage type the
+//    interpolation weights, lerps and accumulation are FP32 and the FP16
+//    volume is the FP32 arithmetic on the decoded samples.  FDK_F16_PACKED
+//    (opt-in) instead lerps in packed FP16 and weights with v_dot2 (FP16
+//    fraction and column weights): faster, outside that precision form;
 //  * z-runs that project entirely inside the detector for every active lane of
 //    a wave (the common case) skip the per-slice range test;
 //  * workgroup ids are remapped so the workgroups an XCD runs together cover a
@@ -36,15 +44,15 @@
 
 // Tuning knobs (defaults chosen by measurement on MI355X, 1024^3 from
 // 1800 x 2048^2):
-//   TKV_BP_ZR_F32 / _F16 / _BF16   slices per thread (register z-run) per
-//                                  storage type (FP64 uses 8)
+//   TKV_BP_ZR_F32 / _F16 / _BF16 / _F16P  slices per thread (register z-run)
+//                                  per storage type (FP64 uses 8)
 //   TKV_BP_LB                      LDS gathers per lane issued together
+//   TKV_BP_GROUP_FENCE             1: no instruction moves between groups of
+//                                  LB gathers
 //   TKV_BP_BOXP_F32 / _F16 / _BF16 LDS row pitch of the staged box in 32-bit
 //                                  words (>= 64, multiple of 4): rows of one
-//                                  column fall in different banks
-//   TKV_BP_F16_PAIRS               FP16 storage keeps FP16 sample pairs in LDS
-//                                  (one 32-bit LDS word per tap pair, lerp
-//                                  along v in packed FP16); 0: FP32 box
+//                                  column fall in different banks (the FP16
+//                                  pitch serves FDK_F16 and FDK_F16_PACKED)
 #ifndef TKV_BP_ZR_F32
 #define TKV_BP_ZR_F32 24
 #endif
@@ -54,8 +62,15 @@
 #ifndef TKV_BP_ZR_BF16
 #define TKV_BP_ZR_BF16 32
 #endif
+#ifndef TKV_BP_ZR_F16P
+#define TKV_BP_ZR_F16P 32
+#endif
 #ifndef TKV_BP_LB
 #define TKV_BP_LB 4
+#endif
+// TKV_BP_GROUP_FENCE: the compiler schedules each group of LB gathers on its own
+#ifndef TKV_BP_GROUP_FENCE
+#define TKV_BP_GROUP_FENCE 1
 #endif
 #ifndef TKV_BP_BOXP_F16
 #define TKV_BP_BOXP_F16 72
@@ -65,9 +80,6 @@
 #endif
 #ifndef TKV_BP_BOXP_BF16
 #define TKV_BP_BOXP_BF16 80
-#endif
-#ifndef TKV_BP_F16_PAIRS
-#define TKV_BP_F16_PAIRS 1
 #endif
 
 namespace tkv {
@@ -90,15 +102,19 @@ struct BPArgs {
     R *vol;
     int Nx, Ny, Nz, Nu, Nv, nang;
     R cX, cZ, D, umax, vmax, out_scale;
+    float one;                    // 1.0 (a run-time value: see gather_mix)
     double maskR2;
     int accumulate;
     int tiles_x, tiles_xy, nblocks;
 };
 
 // ---- storage types ----------------------------------------------------------------
+// PAIRS: the LDS box holds FP16 sample pairs; MIX: they are interpolated with
+// FP32 fractions and weights in mixed-precision FMAs (else packed FP16)
 struct st_f32;
 struct st_f16;
 struct st_bf16;
+struct st_f16p;
 struct st_f64;
 template <typename S>
 struct Store;
@@ -110,6 +126,7 @@ struct Store<st_f32> {
     static constexpr int ZR = TKV_BP_ZR_F32;
     static constexpr int P = TKV_BP_BOXP_F32;
     static constexpr bool PAIRS = false;
+    static constexpr bool MIX = false;
     static __device__ __forceinline__ float get(const char *p) { return *(const float *)p; }
     static __device__ __forceinline__ float half(unsigned, int) { return 0.f; }
 };
@@ -120,7 +137,8 @@ struct Store<st_f16> {
     static constexpr bool STAGED = true;
     static constexpr int ZR = TKV_BP_ZR_F16;
     static constexpr int P = TKV_BP_BOXP_F16;
-    static constexpr bool PAIRS = TKV_BP_F16_PAIRS != 0;
+    static constexpr bool PAIRS = true;
+    static constexpr bool MIX = true;
     static __device__ __forceinline__ float get(const char *p)
     {
         return __half2float(__ushort_as_half(*(const unsigned short *)p));
@@ -138,6 +156,7 @@ struct Store<st_bf16> {
     static constexpr int ZR = TKV_BP_ZR_BF16;
     static constexpr int P = TKV_BP_BOXP_BF16;
     static constexpr bool PAIRS = false;
+    static constexpr bool MIX = false;
     static __device__ __forceinline__ float get(const char *p)
     {
         return __uint_as_float((unsigned)(*(const unsigned short *)p) << 16);
@@ -148,6 +167,11 @@ struct Store<st_bf16> {
     }
 };
 template <>
+struct Store<st_f16p> : Store<st_f16> {
+    static constexpr int ZR = TKV_BP_ZR_F16P;
+    static constexpr bool MIX = false;
+};
+template <>
 struct Store<st_f64> {
     using R = double;
     static constexpr int BYTES = 8;
@@ -155,6 +179,7 @@ struct Store<st_f64> {
     static constexpr int ZR = 8;
     static constexpr int P = 64;   // unused: FP64 is gathered from HBM
     static constexpr bool PAIRS = false;
+    static constexpr bool MIX = false;
     static __device__ __forceinline__ double get(const char *p) { return *(const double *)p; }
     static __device__ __forceinline__ float half(unsigned, int) { return 0.f; }
 };
@@ -192,7 +217,7 @@ __device__ __forceinline__ void gather_lds(float *acc, const float *az, const fl
         const float v = fmaf(az[k], Uv, cz);
         const float fv = __builtin_amdgcn_fractf(v);
         bool ok = true;
-        int off = (int)v * (4 * P) + colb;
+        int off = (int)__umul24((unsigned)(int)v, 4u * P) + colb;   // v >= 0 where used
         if (!FAST) {
             ok = v >= 0.f && v <= vmax;
             off = ok ? off : 0;
@@ -210,10 +235,51 @@ __device__ __forceinline__ void gather_lds(float *acc, const float *az, const fl
         acc[k] = fmaf(okk[k] ? wu1 : 0.f, c1[k], fmaf(okk[k] ? wu0 : 0.f, c0[k], acc[k]));
 }
 
-// FP16 pairs in LDS: word (r, c) holds samples c and c+1 of row r; one
-// ds_read2 fetches both rows of the 2x2 neighbourhood; the lerp along v runs
-// in packed FP16; the two column weights (FP16, normalised to at most U^2) and
-// the two lerped columns meet in one v_dot2_f32_f16, summed in FP32.
+// FP16 pairs in LDS, FP32 interpolation (FDK_F16): word (r, c) holds samples
+// c and c+1 of row r; one ds_read2 fetches both rows of the 2x2
+// neighbourhood.  Each column's difference along v and its lerp are
+// mixed-precision FMAs reading the FP16 halves directly (v_fma_mix_f32: the
+// product is exact and the result rounds once to FP32, so d = q10 - q00 and
+// c = q00 + fv*d equal the FP32 operations on the widened samples); the
+// column weights (FP32, including the projection's scale) accumulate in FP32.
+template <bool FAST, int LB, int P>
+__device__ __forceinline__ void gather_mix(float *acc, const float *az, const float *box,
+                                           float Uv, float cz, int colb, float wu0, float wu1,
+                                           float vmax, float one)
+{
+    const char *lds = (const char *)box;
+    float c0[LB], c1[LB];
+    bool okk[LB];
+#pragma unroll
+    for (int k = 0; k < LB; ++k) {
+        const float v = fmaf(az[k], Uv, cz);
+        const float fv = __builtin_amdgcn_fractf(v);
+        bool ok = true;
+        int off = (int)__umul24((unsigned)(int)v, 4u * P) + colb;   // v >= 0 where used
+        if (!FAST) {
+            ok = v >= 0.f && v <= vmax;
+            off = ok ? off : 0;
+        }
+        const half2_t w0 = *(const half2_t *)(lds + off);          // (q00, q01)
+        const half2_t w1 = *(const half2_t *)(lds + off + 4 * P);  // (q10, q11)
+        // d = q1*one - q0 (one = 1 at run time: an FMA of the widened halves,
+        // which the compiler forms as v_fma_mix_f32), then c = q0 + fv*d
+        const float d0 = fmaf((float)w1.x, one, -(float)w0.x);
+        const float d1 = fmaf((float)w1.y, one, -(float)w0.y);
+        c0[k] = fmaf(fv, d0, (float)w0.x);
+        c1[k] = fmaf(fv, d1, (float)w0.y);
+        okk[k] = ok;
+    }
+    // taps outside the detector get zero weight (the staged values are finite)
+#pragma unroll
+    for (int k = 0; k < LB; ++k)
+        acc[k] = fmaf(okk[k] ? wu1 : 0.f, c1[k], fmaf(okk[k] ? wu0 : 0.f, c0[k], acc[k]));
+}
+
+// FP16 pairs in LDS, packed-FP16 interpolation (FDK_F16_PACKED): the lerp
+// along v runs in packed FP16; the two column weights (FP16, normalised to at
+// most U^2) and the two lerped columns meet in one v_dot2_f32_f16, summed in
+// FP32.
 template <bool FAST, int LB, int P>
 __device__ __forceinline__ void gather_pairs(float *acc, const float *az, const float *box,
                                              float Uv, float cz, int colb, half2_t wuh,
@@ -234,7 +300,7 @@ __device__ __forceinline__ void gather_pairs(float *acc, const float *az, const 
         for (int j = 0; j < 2; ++j) {
             const float v = j ? vb : va;
             bool ok = true;
-            int off = (int)v * (4 * P) + colb;
+            int off = (int)__umul24((unsigned)(int)v, 4u * P) + colb;   // v >= 0 where used
             if (!FAST) {
                 ok = v >= 0.f && v <= vmax;
                 off = ok ? off : 0;
@@ -258,18 +324,23 @@ __device__ __forceinline__ void gather_pairs(float *acc, const float *az, const 
 }
 
 // all ZR slices of a thread for one staged angle, LB gathers at a time
-template <bool FAST, bool PAIRS, int ZR, int LB, int P>
+template <bool FAST, bool PAIRS, bool MIX, int ZR, int LB, int P>
 __device__ __forceinline__ void gather_run(float (&acc)[ZR], const float (&az)[ZR],
                                            const float *box, float Uv, float cz, int colb,
-                                           float wu0, float wu1, half2_t wuh, float vmax)
+                                           float wu0, float wu1, half2_t wuh, float vmax,
+                                           float one)
 {
 #pragma unroll
     for (int k0 = 0; k0 < ZR; k0 += LB) {
-        if constexpr (PAIRS)
+        if constexpr (PAIRS && MIX)
+            gather_mix<FAST, LB, P>(acc + k0, az + k0, box, Uv, cz, colb, wu0, wu1, vmax, one);
+        else if constexpr (PAIRS)
             gather_pairs<FAST, LB, P>(acc + k0, az + k0, box, Uv, cz, colb, wuh, vmax);
         else
             gather_lds<FAST, LB, P>(acc + k0, az + k0, box, Uv, cz, colb, wu0, wu1, vmax);
+#if TKV_BP_GROUP_FENCE
         __builtin_amdgcn_sched_barrier(0);
+#endif
     }
 }
 
@@ -403,11 +474,12 @@ __global__ void __launch_bounds__(256) bp_kernel(BPArgs<typename Store<S>::R> a)
 #pragma unroll
     for (int k = 0; k < ZR; ++k) acc[k] = 0;
 
-    // FP16 pairs carry the column weights in FP16: they are normalised by the
-    // largest projection scale (rounded up to a power of two, g), so they stay
-    // within [0, U^2], and the sums are multiplied by g at the end
+    // packed FP16 carries the column weights in FP16: they are normalised by
+    // the largest projection scale (rounded up to a power of two, g), so they
+    // stay within [0, U^2], and the sums are multiplied by g at the end
+    constexpr bool PACKED = ST::PAIRS && !ST::MIX;
     R wnorm = 1, wback = 1;
-    if constexpr (ST::PAIRS) {
+    if constexpr (PACKED) {
         float m = 0.f;
         for (int i = lane; i < a.nang; i += 64) m = fmaxf(m, (float)a.scales[i]);
 #pragma unroll
@@ -490,12 +562,13 @@ __global__ void __launch_bounds__(256) bp_kernel(BPArgs<typename Store<S>::R> a)
                 constexpr bool FAST = decltype(fast_tag)::value;
                 const R c = a.ang[2 * b], s = a.ang[2 * b + 1];
                 // the reference's S, T, U and u with nvcc's contractions; the
-                // FP16 path, whose error is set by its storage, takes U from the
-                // hardware reciprocal instead of the IEEE quotient
+                // packed-FP16 path, whose error is set by its FP16 weights,
+                // takes U from the hardware reciprocal instead of the IEEE
+                // quotient
                 const R Sx = rfma(aX, c, aY * s);
                 const R T = rfma(aY, c, -(aX * s));
                 R U;
-                if constexpr (ST::PAIRS)
+                if constexpr (PACKED)
                     U = a.D * __builtin_amdgcn_rcpf(a.D - Sx);
                 else
                     U = a.D / (a.D - Sx);
@@ -506,14 +579,14 @@ __global__ void __launch_bounds__(256) bp_kernel(BPArgs<typename Store<S>::R> a)
                 const R W = act ? U * U * (a.scales[b] * wnorm) : (R)0;
                 const R wu1 = W * fu, wu0 = W - wu1;
                 half2_t wuh = {0, 0};
-                if constexpr (ST::PAIRS) wuh = half2_t{(_Float16)wu0, (_Float16)wu1};
+                if constexpr (PACKED) wuh = half2_t{(_Float16)wu0, (_Float16)wu1};
                 // LDS byte offset of sample (row iv, column iu) is
                 // iv*4*P + colb; inactive lanes sit at v = bv0, offset 0
                 const int colb = (act ? (iu - bx.x) * 4 : 0) - bx.y * (4 * P);
                 const R Uv = act ? U : (R)0;
                 const R cz = act ? a.cZ : (R)bx.y;
-                gather_run<FAST, ST::PAIRS, ZR, LB, P>(acc, az, boxmem, Uv, cz, colb, wu0, wu1,
-                                                       wuh, a.vmax);
+                gather_run<FAST, ST::PAIRS, ST::MIX, ZR, LB, P>(acc, az, boxmem, Uv, cz, colb,
+                                                                wu0, wu1, wuh, a.vmax, a.one);
             };
             using fast_t = std::integral_constant<bool, true>;
             using edge_t = std::integral_constant<bool, false>;
@@ -638,6 +711,7 @@ int bp_zrun(int dtype)
     switch (dtype) {
     case FDK_F32: return Store<st_f32>::ZR;
     case FDK_F16: return Store<st_f16>::ZR;
+    case FDK_F16_PACKED: return Store<st_f16p>::ZR;
     case FDK_BF16: return Store<st_bf16>::ZR;
     case FDK_F64: return Store<st_f64>::ZR;
     default: return 0;
@@ -649,6 +723,7 @@ int sample_bytes_of(int dtype)
     switch (dtype) {
     case FDK_F32: return 4;
     case FDK_F16: return 2;
+    case FDK_F16_PACKED: return 2;
     case FDK_BF16: return 2;
     case FDK_F64: return 8;
     default: return 0;
@@ -686,6 +761,7 @@ static int launch_bp_t(const BPLaunch &L, hipStream_t st)
     a.umax = (R)(L.Nu - 1);
     a.vmax = (R)(L.Nv - 1);
     a.out_scale = (R)L.out_scale;
+    a.one = 1.0f;
     a.maskR2 = L.maskR2;
     a.accumulate = L.accumulate;
     a.tiles_x = (L.Nx + 15) / 16;
@@ -706,6 +782,7 @@ int launch_backproject(int dtype, const BPLaunch &a, hipStream_t st)
     switch (dtype) {
     case FDK_F32: return launch_bp_t<st_f32>(a, st);
     case FDK_F16: return launch_bp_t<st_f16>(a, st);
+    case FDK_F16_PACKED: return launch_bp_t<st_f16p>(a, st);
     case FDK_BF16: return launch_bp_t<st_bf16>(a, st);
     case FDK_F64: return launch_bp_t<st_f64>(a, st);
     default: return fail(FDK_ERR_INVALID_ARGUMENT, "backproject: unknown storage type");

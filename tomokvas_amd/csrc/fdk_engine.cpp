@@ -726,8 +726,8 @@ int fdk_set_filter(fdk_ctx *c, int kind, const double *taps, int ntaps)
 int fdk_set_storage(fdk_ctx *c, int dtype)
 {
     if (!c) return fail(FDK_ERR_INVALID_ARGUMENT, "null context");
-    if (dtype != FDK_F32 && dtype != FDK_F16 && dtype != FDK_BF16)
-        return fail(FDK_ERR_INVALID_ARGUMENT, "storage must be F32, F16 or BF16");
+    if (dtype != FDK_F32 && dtype != FDK_F16 && dtype != FDK_BF16 && dtype != FDK_F16_PACKED)
+        return fail(FDK_ERR_INVALID_ARGUMENT, "storage must be F32, F16, BF16 or F16_PACKED");
     if (c->initialized && dtype != c->storage) {
         int rc = ensure(&c->d_stack, c->cap_stack, fdk_stack_bytes(c, dtype, c->g.Nang),
                         "stored stack");

@@ -840,6 +840,7 @@ int launch_pack(int dtype_in, const void *filt, int Nu, int Nv, int nang, int pi
                                Nv, nang, pitch, maxabs, (float4 *)stack, (float *)scales);
             break;
         case FDK_F16:
+        case FDK_F16_PACKED:   // same stored stack; only the backprojector differs
             if (!maxabs) return fail(FDK_ERR_INVALID_ARGUMENT, "pack: FP16 needs maxabs");
             hipLaunchKernelGGL((pack_kernel<float, f16_t, true>), grid, dim3(256), 0, st, f, Nu,
                                Nv, nang, pitch, maxabs, (uint2 *)stack, (float *)scales);

@@ -26,6 +26,7 @@ int main(int argc, char **argv)
             if (t == "f32") opt.storage = FDK_F32;
             else if (t == "f16") opt.storage = FDK_F16;
             else if (t == "bf16") opt.storage = FDK_BF16;
+            else if (t == "f16-packed") opt.storage = FDK_F16_PACKED;
             else {
                 std::fprintf(stderr, "unknown storage type %s\n", t.c_str());
                 return 2;
