@@ -17,6 +17,7 @@ import ctypes as C
 import json
 import math
 import os
+import subprocess
 import sys
 import time
 
@@ -227,10 +228,22 @@ def read_profile(workload: str, storage: str):
     return None, None
 
 
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_leg(fbp, w, g, block_vols, hipm, threads):
-    """CPU baseline (FP32 mirror, timed) and FP64 oracle on a seeded 64^3 block."""
+    """CPU baseline: the FP32 OpenMP mirror (timed) and the FP64 oracle on a
+    seeded 64^3 block of the workload (its detector rows only), and the cfg1
+    workload (128^3 from 180 x 128^2) reconstructed in full by the mirror."""
     import oracle as O
-    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    os.environ["OMP_NUM_THREADS"] = str(threads)
     x0, y0, z0, n = geometry.sub_cube(g, w.seed)
     lo, hi = geometry.detector_rows_of_block(g, x0, y0, z0, n)
     rows = hi - lo
@@ -246,25 +259,50 @@ def cpu_leg(fbp, w, g, block_vols, hipm, threads):
                              np.arange(x0, x0 + n), indexing="ij")
     pts = np.stack([xx.ravel(), yy.ravel(), zz.ravel()], axis=1).astype(np.int32)
     Hf = O.filter_spectrum(w.filter, g.Nu, g.tau)
-    # FP32 mirror: the timed CPU baseline (weight + filter of the rows + BP of the block)
-    t0 = time.perf_counter()
-    f32 = O.filter_rows(og, O.weight(og, raw), Hf)
-    t1 = time.perf_counter()
-    m32 = O.backproject_points(og, f32, pts)
-    t2 = time.perf_counter()
+    # FP32 mirror: the timed CPU baseline (weight + filter of the rows + BP of
+    # the block), median of 3
+    tf, tb = [], []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        f32 = O.filter_rows(og, O.weight(og, raw), Hf)
+        t1 = time.perf_counter()
+        m32 = O.backproject_points(og, f32, pts)
+        t2 = time.perf_counter()
+        tf.append(t1 - t0)
+        tb.append(t2 - t1)
+    t_f, t_b = float(np.median(tf)), float(np.median(tb))
     # FP64 oracle on the same raw projections
+    t2 = time.perf_counter()
     f64 = O.filter_rows(og, O.weight(og, raw.astype(np.float64), f64=True), Hf, f64=True)
     o64 = O.backproject_points(og, f64, pts, f64=True)
     t3 = time.perf_counter()
     del f32, f64, raw
     upd = pts.shape[0] * g.Nang
+    gups = upd / t_b / 1e9
+    # the whole workload, extrapolated per voxel-update and per filtered row
+    bp_full = g.voxel_updates / (gups * 1e9)
+    filt_full = t_f * (g.Nv / rows)
+    # cfg1 in full: the mirror's FDK of the Shepp-Logan projections
+    w1 = geometry.workloads()["cfg1"]
+    g1 = w1.geom
+    o1 = O.Geometry(**g1.oracle_dict())
+    s1 = O.project_ellipsoids(o1, O.shepp_logan(g1.fov_px / 2)).astype(np.float32)
+    t4 = time.perf_counter()
+    O.fdk(o1, s1, kind=0)
+    t5 = time.perf_counter()
     out = {
-        "value": upd / (t2 - t1) / 1e9, "unit": "GUPS", "cores": threads, "kind": "port",
-        "sample": (f"FP32 OpenMP mirror, backprojection of a seeded {n}^3 voxel block "
-                   f"(x{x0} y{y0} z{z0}) over all {g.Nang} projections = {upd:.3e} voxel-updates; "
-                   f"weight+filter of its {rows} detector rows x {g.Nang} projections took "
-                   f"{t1 - t0:.2f} s"),
-        "bp_seconds": t2 - t1, "filter_seconds": t1 - t0, "fp64_oracle_seconds": t3 - t2,
+        "value": gups, "unit": "GUPS", "cores": threads, "kind": "port",
+        "nproc": os.cpu_count(), "cpu_model": cpu_model(),
+        "sample": (f"FP32 OpenMP mirror (gcc -O3 -march=native -fopenmp, {threads} threads), "
+                   f"backprojection of a seeded {n}^3 voxel block (x{x0} y{y0} z{z0}) over all "
+                   f"{g.Nang} projections = {upd:.3e} voxel-updates, median of 3; weight+filter "
+                   f"of its {rows} detector rows x {g.Nang} projections {t_f:.2f} s"),
+        "bp_seconds": t_b, "filter_seconds": t_f, "fp64_oracle_seconds": t3 - t2,
+        "extrapolated_full_fdk_seconds": bp_full + filt_full,
+        "extrapolated_full_bp_seconds": bp_full,
+        "extrapolated_full_filter_seconds": filt_full,
+        "cfg1_full_fdk_seconds": t5 - t4,
+        "cfg1_gups": g1.voxel_updates / (t5 - t4) / 1e9,
     }
     errs = {"mirror_f32_vs_fp64": O.rel_rmse(m32, o64)}
     for st, vol in block_vols.items():
@@ -275,6 +313,130 @@ def cpu_leg(fbp, w, g, block_vols, hipm, threads):
     return out, errs, (x0, y0, z0, n)
 
 
+def fdk_steps(fbp, L, hipm, steps, warmup):
+    """Mean stage times [filter, pack, BP] (ms) of `steps` FDK passes, and the
+    mean wall time per pass."""
+    nullcb = _lib.GET_BLOCK_FN()
+    for _ in range(warmup):
+        _lib.check(L.fdk_calculate_reconstruction(fbp.handle, nullcb, None), "warmup")
+    hipm.hipDeviceSynchronize()
+    walls, stages = [], []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        _lib.check(L.fdk_calculate_reconstruction(fbp.handle, nullcb, None), "step")
+        hipm.hipDeviceSynchronize()
+        walls.append(time.perf_counter() - t0)
+        stages.append(fbp.stage_times_ms())
+    s = np.array(stages)
+    return [float(np.mean(s[:, i])) for i in range(3)], 1e3 * float(np.mean(walls))
+
+
+def config_leg(name, hipm, storages, steps):
+    """GUPS and FDK seconds of one workload per storage type, with each
+    low-precision volume's relative RMSE against the FP32 volume (on the GPU)."""
+    L = _lib.load()
+    w = geometry.workloads()[name]
+    g = w.geom
+    fbp = FilteredBackProjection(0)
+    g.apply(fbp, None, None)
+    _lib.check(fbp.SetFilter(w.filter), "SetFilter")
+    _lib.check(fbp.SetStorage(_lib.F32), "SetStorage")
+    _lib.check(fbp.Initialize(), "Initialize")
+    ells = np.ascontiguousarray(w.ellipsoids())
+    _lib.check(L.fdk_phantom_project(fbp.handle, ells.ctypes.data, ells.shape[0], 0, g.Nang,
+                                     fbp.device_sinogram_ptr(), None), "phantom")
+    if w.I0:
+        _lib.check(L.fdk_poisson_log(fbp.device_sinogram_ptr(), g.Nu * g.Nv * g.Nang, w.I0,
+                                     w.seed, None), "poisson")
+    nvol = g.Nx * g.Ny * g.Nz
+    ref = C.c_void_p()
+    assert hipm.hipMalloc(C.byref(ref), nvol * 4) == 0
+    out = {"workload": w.name, "voxel_updates": g.voxel_updates}
+    for st in storages:
+        _lib.check(fbp.SetStorage(st), "SetStorage")
+        (f, pk, bp), wall = fdk_steps(fbp, L, hipm, steps, 1)
+        dvol = C.c_void_p(fbp.device_volume_ptr())
+        e = None
+        if st == _lib.F32:
+            assert hipm.hipMemcpy(ref, dvol, nvol * 4, 3) == 0
+        else:
+            o3 = (C.c_double * 3)()
+            _lib.check(L.fdk_compare(dvol, ref, nvol, o3, None), "compare")
+            e = math.sqrt(o3[0] / o3[1])
+        out[NAMES[st]] = {"gups": g.voxel_updates / (bp * 1e-3) / 1e9, "fdk_s": (f + pk + bp) / 1e3,
+                          "filter_ms": f, "pack_ms": pk, "bp_ms": bp,
+                          "rel_rmse_vs_f32_volume": e}
+    hipm.hipFree(ref)
+    fbp.close()
+    log(f"{name}: " + ", ".join(f"{k} {v['fdk_s'] * 1e3:.2f} ms ({v['gups']:.0f} GUPS)"
+                                for k, v in out.items() if isinstance(v, dict)))
+    return out
+
+
+def diff_leg(name, steps_chunked):
+    """One training step of the differentiable form (tomokvas_amd.diff): loss =
+    sum of squares over the central 256^3 crop of the volume (all of it below
+    256^3), gradients w.r.t. the projections and the p learnable taps.  Reports
+    step time, peak device memory the step allocates above its inputs, and the
+    gradient error of each variant against the FP32 step without chunks."""
+    import torch
+    from tomokvas_amd import diff
+    from tomokvas_amd.stages import Stages
+    w = geometry.workloads()[name]
+    g = w.geom
+    fbp = FilteredBackProjection(0)
+    g.apply(fbp, None, None)
+    st = Stages(fbp)
+    sino = st.phantom(w.ellipsoids())
+    taps = diff.ramlak_taps(st)
+    diff.set_taps(st, taps)
+    Nz, Ny, Nx = st.vol_shape
+    c = [min(128, n // 2) for n in (Nz, Ny, Nx)]
+    crop = tuple(slice(n // 2 - h, n // 2 + h) for n, h in zip((Nz, Ny, Nx), c))
+
+    def step(storage, chunk):
+        vol = diff.forward(st, sino, storage=storage, chunk=chunk)
+        gvol = torch.zeros_like(vol)
+        gvol[crop] = 2 * vol[crop]
+        loss = float((vol[crop].double() ** 2).sum())
+        del vol
+        gp, dh = diff.backward(st, sino, gvol, chunk=chunk)
+        return loss, gp, dh
+
+    out = {"workload": w.name, "taps": st.p, "crop": [2 * h for h in c]}
+    grads = {}
+    for label, storage, chunk in steps_chunked:
+        step(storage, chunk)                     # warm-up (code objects, allocator)
+        torch.cuda.synchronize()
+        torch.cuda.reset_peak_memory_stats()
+        base = torch.cuda.memory_allocated()
+        t0 = time.perf_counter()
+        loss, gp, dh = step(storage, chunk)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        peak = torch.cuda.max_memory_allocated() - base
+        grads[label] = (gp, dh)
+        out[label] = {"storage": NAMES[storage], "chunk": chunk, "step_s": dt,
+                      "peak_alloc_gib": peak / 2**30,
+                      "inputs_gib": sino.numel() * 4 / 2**30, "loss": loss}
+    ref_gp, ref_dh = grads["f32_whole"]
+    for label, (gp, dh) in grads.items():
+        if label == "f32_whole":
+            continue
+        out[label]["grad_proj_rel_err_vs_f32_whole"] = float(
+            torch.linalg.vector_norm((gp - ref_gp).double()) /
+            torch.linalg.vector_norm(ref_gp.double()))
+        out[label]["grad_taps_rel_err_vs_f32_whole"] = float(
+            np.linalg.norm(dh - ref_dh) / np.linalg.norm(ref_dh))
+    del grads, sino
+    fbp.close()
+    torch.cuda.empty_cache()
+    log(f"{name} differentiation: " + ", ".join(
+        f"{k} {v['step_s']:.3f} s / {v['peak_alloc_gib']:.1f} GiB"
+        for k, v in out.items() if isinstance(v, dict)))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -283,6 +445,10 @@ def main():
     ap.add_argument("--workload", default="cfg3")
     ap.add_argument("--storages", default="f32,f16,bf16,f16-packed")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="leave out the other workloads' GUPS/FDK legs")
+    ap.add_argument("--no-diff", action="store_true",
+                    help="leave out the differentiation-form training steps")
     ap.add_argument("--rsq", action="store_true",
                     help="also time the RSQ preprocessing (248 sinograms of 2000 x 1024)")
     ap.add_argument("--tiff", metavar="DIR",
@@ -292,12 +458,22 @@ def main():
     ap.add_argument("--no-slab-pipeline", action="store_true",
                     help="leave out the run with every slab handed to host memory (its chunked "
                          "backprojection launches would mix into a kernel trace's averages)")
+    # the host CPU share of one GPU on the MI355X machines this runs on is 16
+    # CPUs (nproc may report the whole machine)
     ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--calibrate", action="store_true",
                     help="after the timed steps, run accumulate_kernel once over 2^28 floats "
                          "(reads 2 GiB, writes 1 GiB): the reference launch for scaling "
                          "rocprofv3 FETCH_SIZE/WRITE_SIZE readings to bytes")
+    ap.add_argument("--diff-leg", metavar="WORKLOAD",
+                    help="only run the differentiation-form training steps of WORKLOAD "
+                         "(cfg4 or cfg5) and print their JSON")
     args = ap.parse_args()
+    if args.diff_leg:
+        import torch
+        assert torch.cuda.is_available(), "the differentiation leg needs a HIP device"
+        print(json.dumps(diff_leg(args.diff_leg, DIFF_STEPS[args.diff_leg])), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the FDK path "
               "is not sharded across GPUs in this build", file=sys.stderr)
@@ -328,7 +504,6 @@ def main():
     nvol = g.Nx * g.Ny * g.Nz
     ref = C.c_void_p()
     assert hipm.hipMalloc(C.byref(ref), nvol * 4) == 0
-    nullcb = _lib.GET_BLOCK_FN()
     order = [CODES[s] for s in args.storages.split(",")]
     if _lib.F32 in order:
         order.remove(_lib.F32)
@@ -338,18 +513,7 @@ def main():
     block_vols = {}
     for st in order:
         _lib.check(fbp.SetStorage(st), "SetStorage")
-        for _ in range(args.warmup):
-            _lib.check(L.fdk_calculate_reconstruction(fbp.handle, nullcb, None), "warmup")
-        hipm.hipDeviceSynchronize()
-        walls, stages = [], []
-        for _ in range(args.steps):
-            t0 = time.perf_counter()
-            _lib.check(L.fdk_calculate_reconstruction(fbp.handle, nullcb, None), "step")
-            hipm.hipDeviceSynchronize()
-            walls.append(time.perf_counter() - t0)
-            stages.append(fbp.stage_times_ms())
-        s = np.array(stages)
-        filt_ms, pack_ms, bp_ms = (float(np.mean(s[:, i])) for i in range(3))
+        (filt_ms, pack_ms, bp_ms), wall_ms = fdk_steps(fbp, L, hipm, args.steps, args.warmup)
         fdk_ms = filt_ms + pack_ms + bp_ms
         dvol = fbp.device_volume_ptr()
         cmp = None
@@ -365,8 +529,7 @@ def main():
                               slab.nbytes, 2) == 0
         block_vols[NAMES[st]] = slab
         res[NAMES[st]] = dict(filter_ms=filt_ms, pack_ms=pack_ms, bp_ms=bp_ms, fdk_ms=fdk_ms,
-                              wall_ms=1e3 * float(np.mean(walls)),
-                              gups=g.voxel_updates / (bp_ms * 1e-3) / 1e9,
+                              wall_ms=wall_ms, gups=g.voxel_updates / (bp_ms * 1e-3) / 1e9,
                               rel_rmse_vs_f32=cmp)
         log(f"{NAMES[st]}: filter {filt_ms:.2f} ms, pack {pack_ms:.2f} ms, BP {bp_ms:.2f} ms "
             f"({res[NAMES[st]]['gups']:.1f} GUPS), FDK {fdk_ms:.2f} ms, vs f32 {cmp}")
@@ -379,61 +542,77 @@ def main():
         pipe["fdk_without_handout_ms"] = r0["fdk_ms"]
         pipe["fdk_with_handout_ms"] = pipe["filter_ms"] + pipe["pack_ms"] + pipe["bp_ms"] + \
             pipe["exposed_transfer_ms"]
-    if pipe:
         log(f"slab hand-out ({pipe['storage']}, chunks of {pipe['chunk_slices']} slices): BP "
-        f"{pipe['bp_ms']:.2f} ms, exposed transfers {pipe['exposed_transfer_ms']:.2f} ms, "
-        f"wall {pipe['wall_ms']:.1f} ms, slabs equal device volume: "
-        f"{pipe['slabs_equal_device_volume']}")
+            f"{pipe['bp_ms']:.2f} ms, exposed transfers {pipe['exposed_transfer_ms']:.2f} ms, "
+            f"wall {pipe['wall_ms']:.1f} ms, slabs equal device volume: "
+            f"{pipe['slabs_equal_device_volume']}")
 
+    # the headline: FP16 storage with FP32 interpolation weights and accumulation
     head = "f16" if "f16" in res else NAMES[order[0]]
     r = res[head]
     upd = g.voxel_updates
     s_bytes = CELL_BYTES[CODES[head]]
     gather = 4 * s_bytes * upd                      # 2x2 taps per voxel-update
-    achieved = gather / (r["bp_ms"] * 1e-3) / 1e9   # GB/s
+    gather_gbs = gather / (r["bp_ms"] * 1e-3) / 1e9
     traffic, valu_per_update = read_profile(args.workload, head)
     t_bp = r["bp_ms"] * 1e-3
     nlaunch = C.c_int()
     _lib.check(L.fdk_get_angle_chunks(fbp.handle, C.byref(nlaunch)), "angle chunks")
     # one step backprojects the angle loop in K launches of Nang/K angles each:
-    # per-launch figures are the step's divided by K (achieved is unchanged);
-    # the traffic profile holds the step's bytes (its K dispatches summed)
+    # per-launch figures are the step's divided by K; the traffic profile holds
+    # the step's bytes (its K dispatches summed)
     K = nlaunch.value
     roofline = {
-        "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": achieved / HBM_PEAK_GBS, "traffic": traffic / K if traffic else None,
-        "kernel": f"bp_kernel<{head}>", "algorithmic_bytes": gather / K,
-        "launches_per_step": K, "avg_launch_ms": r["bp_ms"] / K,
-        "l2_peak": L2_PEAK_GBS, "frac_l2": achieved / L2_PEAK_GBS,
+        "kernel": f"bp_kernel<{head}>", "launches_per_step": K, "avg_launch_ms": r["bp_ms"] / K,
+        "traffic": traffic / K if traffic else None,
+        "algorithmic_bytes": gather / K,
         "compulsory_hbm_bytes_per_step": g.Nang * g.Nv * g.Nu * s_bytes + 4 * g.Nx * g.Ny * g.Nz,
-        "hbm_traffic_rate_gbs": traffic / t_bp / 1e9 if traffic else None,
     }
-    # the L2 figure measured on this device (2 MiB, resident in every XCD's L2)
+    if valu_per_update:
+        # the gathers are served from LDS and the kernel is bound by VALU issue:
+        # achieved = lane-instructions per second (counter-measured instructions
+        # per voxel-update x updates / BP time), peak = 256 CUs x 64 lanes x 2.4 GHz
+        ach = upd * valu_per_update / t_bp
+        roofline.update({"bound": "valu", "achieved": ach / 1e12, "peak": VALU_LANE_OPS / 1e12,
+                         "unit": "T lane-instr/s", "frac": ach / VALU_LANE_OPS,
+                         "valu_instr_per_update": valu_per_update})
+    else:
+        roofline.update({"bound": "valu", "achieved": None, "peak": VALU_LANE_OPS / 1e12,
+                         "unit": "T lane-instr/s", "frac": None})
+    # HBM: the counter-measured bytes against the datasheet peak; L2: the
+    # algorithmic 2x2-gather bytes (4 samples per update) against the L2 figure,
+    # datasheet estimate and measured on this device
+    roofline["hbm_gbs"] = traffic / t_bp / 1e9 if traffic else None
+    roofline["hbm_frac"] = traffic / t_bp / 1e9 / HBM_PEAK_GBS if traffic else None
+    roofline["gather_gbs"] = gather_gbs
+    roofline["l2_peak_est_gbs"] = L2_PEAK_GBS
+    roofline["frac_l2_est"] = gather_gbs / L2_PEAK_GBS
     l2_gbs = C.c_double(0.0)
     if L.fdk_probe_l2_read(2 << 20, C.byref(l2_gbs)) == 0 and l2_gbs.value > 0:
-        roofline["l2_measured"] = l2_gbs.value
-        roofline["frac_l2_measured"] = achieved / l2_gbs.value
+        roofline["l2_measured_gbs"] = l2_gbs.value
+        roofline["frac_l2_measured"] = gather_gbs / l2_gbs.value
     else:
         log(f"L2 probe failed: {_lib.last_error()}")
-    if valu_per_update:
-        # the gathers are served from LDS: the kernel is bound by VALU issue
-        t_valu = upd * valu_per_update / VALU_LANE_OPS
-        roofline["valu"] = {"instr_per_update": valu_per_update,
-                            "peak_lane_instr_per_s": VALU_LANE_OPS,
-                            "t_min_ms": 1e3 * t_valu, "frac": t_valu / t_bp}
+    f32ms = res["f32"]["fdk_ms"] if "f32" in res else None
     out = {
         "metric": METRIC, "value": r["gups"], "unit": "GUPS", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": r["fdk_ms"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": f"{head}-store/f32-accumulate",
+        "dtype": f"{head}-store/f32-interp/f32-accumulate" if head != "f16-packed"
+                 else "f16-store/f16-interp/f32-accumulate",
         "data": "synthetic: 64 seeded random ellipsoids, exact line integrals + Poisson(I0=1e5) "
                 "counts, generated on the GPU",
         "config": w.describe(),
         "fdk_seconds": {k: v["fdk_ms"] / 1e3 for k, v in res.items()},
         "bp_gups": {k: v["gups"] for k, v in res.items()},
         "stage_ms": res,
-        "speedup_fdk_f16_vs_f32": (res["f32"]["fdk_ms"] / res["f16"]["fdk_ms"]
-                                   if "f16" in res and "f32" in res else None),
+        "speedup_fdk_f16_vs_f32": (f32ms / res["f16"]["fdk_ms"]
+                                   if "f16" in res and f32ms else None),
+        "speedup_fdk_bf16_vs_f32": (f32ms / res["bf16"]["fdk_ms"]
+                                    if "bf16" in res and f32ms else None),
+        # opt-in, outside the FP32-interpolation precision form
+        "speedup_fdk_f16_packed_vs_f32": (f32ms / res["f16-packed"]["fdk_ms"]
+                                          if "f16-packed" in res and f32ms else None),
         "rel_rmse": {f"{k}_vs_f32_full_volume": v["rel_rmse_vs_f32"] for k, v in res.items()
                      if v["rel_rmse_vs_f32"] is not None},
         "roofline": roofline,
@@ -475,7 +654,27 @@ def main():
         out["calibration"] = {"kernel": "accumulate_kernel", "floats": n,
                               "read_bytes": 8 * n, "write_bytes": 4 * n}
     fbp.close()
+    if not args.no_configs:
+        out["configs"] = {name: config_leg(name, hipm, order, min(args.steps, 3))
+                          for name in ("cfg1", "cfg2", "cfg4", "cfg5") if name != args.workload}
+    if not args.no_diff:
+        # the training steps run on torch tensors: in a child process that
+        # initialises torch before anything else touches the GPU
+        out["differentiation"] = {}
+        for name in ("cfg4", "cfg5"):
+            r = subprocess.run([sys.executable, os.path.abspath(__file__), "--diff-leg", name],
+                               capture_output=True, text=True, timeout=300)
+            sys.stderr.write(r.stderr)
+            if r.returncode != 0:
+                raise RuntimeError(f"differentiation leg {name} failed ({r.returncode})")
+            out["differentiation"][name] = json.loads(r.stdout.strip().splitlines()[-1])
     print(json.dumps(out), flush=True)
+
+
+DIFF_STEPS = {
+    "cfg4": [("f32_whole", _lib.F32, None), ("f32_chunk50", _lib.F32, 50)],
+    "cfg5": [("f32_whole", _lib.F32, None), ("f16_chunk50", _lib.F16, 50)],
+}
 
 
 if __name__ == "__main__":

@@ -25,11 +25,10 @@ def fma(a, b, c):  # float32 fma via float64 (exact product)
 
 BOX_W, ALIGN = 64, 4
 # the footprint's row limit (bh < limit, else the HBM gather) per kernel
-# configuration (z-run, limit): the backprojector's FP32-pair box of 64 rows
-# (24-slice runs, limit 65: bh <= 64 entry rows), its packed-FP16 box
-# (box_rows<32>() = 96), the adjoint's 64-slice box (box_rows<64>() = 128), and
-# the other box_rows<ZR>() sizes
-KERNELS = {"bp_pairs64_zr24": (24, 65), "bp_f16packed_zr32": (32, 96), "fp_zr64": (64, 128),
+# configuration (z-run, box_rows<ZR>() of the kernel): the backprojector's FP32
+# box (24-slice runs), its FP16 and BF16 boxes (32), the adjoint's 64-slice box,
+# and the other box sizes
+KERNELS = {"bp_f32_zr24": (24, 80), "bp_f16_bf16_zr32": (32, 96), "fp_zr64": (64, 128),
            "zr16": (16, 64), "zr48": (48, 128)}
 
 

@@ -1116,7 +1116,10 @@ int fdk_tap_gradient(fdk_ctx *c, int dtype, const void *d_q, const void *d_G, in
     long long need = ((rows + 1) / 2 + nt - 1) / nt;
     const int nblocks = (int)std::max<long long>(1, std::min<long long>(need, 2048));
     const size_t rb = dtype == FDK_F64 ? 8 : 4;
-    const size_t bytes = (size_t)nblocks * p * 2 * rb;
+    // partial spectra of the workgroups, then their sum (p complex doubles)
+    const size_t pbytes = (size_t)nblocks * p * 2 * rb;
+    const size_t sbytes = (size_t)p * 2 * sizeof(double);
+    const size_t bytes = pbytes + sbytes;
     if (bytes > c->scratch_bytes) {
         dfree(c->d_scratch);
         c->scratch_bytes = 0;
@@ -1133,43 +1136,21 @@ int fdk_tap_gradient(fdk_ctx *c, int dtype, const void *d_q, const void *d_G, in
     L.tw = dtype == FDK_F64 ? (const void *)c->d_tw64 : (const void *)c->d_tw32;
     L.partial = c->d_scratch;
     L.nblocks = nblocks;
+    L.spectrum = (char *)c->d_scratch + pbytes;
     if ((rc = tkv::launch_tap_gradient((int)rb, L, st))) return rc;
-    std::vector<char> host(bytes);
-    TKV_HIP(hipMemcpyAsync(host.data(), c->d_scratch, bytes, hipMemcpyDeviceToHost, st));
+    std::vector<double> P(2 * (size_t)p);
+    TKV_HIP(hipMemcpyAsync(P.data(), L.spectrum, sbytes, hipMemcpyDeviceToHost, st));
     TKV_HIP(hipStreamSynchronize(st));
-    // sum the partial spectra in double, take the Hermitian part, final DFT
-    std::vector<double> Pr(p, 0.), Pi(p, 0.);
-    for (int b = 0; b < nblocks; ++b)
-        for (int k = 0; k < p; ++k) {
-            size_t idx = ((size_t)b * p + k) * 2;
-            if (rb == 8) {
-                const double *h = (const double *)host.data();
-                Pr[k] += h[idx];
-                Pi[k] += h[idx + 1];
-            } else {
-                const float *h = (const float *)host.data();
-                Pr[k] += h[idx];
-                Pi[k] += h[idx + 1];
-            }
-        }
-    std::vector<double> Rr(p), Ri(p), cs(p), sn(p);
+    // the Hermitian part R[k] = (P[k] + conj(P[-k]))/2, then
+    // dh[j] += Re sum_k R[k] e^{-2 pi i k j / p}
+    std::vector<double> Rr(p), Ri(p);
     for (int k = 0; k < p; ++k) {
-        int m = (p - k) % p;
-        Rr[k] = 0.5 * (Pr[k] + Pr[m]);
-        Ri[k] = 0.5 * (Pi[k] - Pi[m]);
-        double a = 2.0 * M_PI * (double)k / (double)p;
-        cs[k] = std::cos(a);
-        sn[k] = std::sin(a);
+        const int m = (p - k) % p;
+        Rr[k] = 0.5 * (P[2 * k] + P[2 * m]);
+        Ri[k] = 0.5 * (P[2 * k + 1] - P[2 * m + 1]);
     }
-    for (int j = 0; j < p; ++j) {
-        double s = 0.;
-        for (int k = 0; k < p; ++k) {
-            int idx = (int)(((long long)k * j) % p);
-            // Re(R[k] e^{-2 pi i k j / p})
-            s += Rr[k] * cs[idx] + Ri[k] * sn[idx];
-        }
-        dh[j] += s;
-    }
+    tkv::fft_pow2(Rr, Ri);
+    for (int j = 0; j < p; ++j) dh[j] += Rr[j];
     return FDK_OK;
 }
 

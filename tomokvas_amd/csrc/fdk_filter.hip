@@ -617,8 +617,9 @@ int launch_filter(int real_bytes, const FilterLaunch &a, hipStream_t st)
 // Two rows A, B travel as one complex sequence (Zq = Q_A + i Q_B, likewise ZG);
 // then Q_A conj(G_A) + Q_B conj(G_B) = (P[k] + conj(P[-k]))/2 with
 // P[k] = Zq[k] conj(ZG[k]).  Each workgroup accumulates P over its row pairs in
-// registers and writes one partial spectrum; the host sums the partials in
-// double, symmetrises and applies the final length-p DFT.
+// registers and writes one partial spectrum; tapgrad_reduce_kernel sums the
+// partials in double, and the host symmetrises and applies the final length-p
+// transform (an FFT in double).
 template <typename R, int P>
 __global__ void __launch_bounds__(256)
     tapgrad_kernel(const R *__restrict__ q, const R *__restrict__ G, int Nu, long long rows,
@@ -680,6 +681,24 @@ __global__ void __launch_bounds__(256)
     }
 }
 
+// Sum of the workgroups' partial spectra, in double: out[k] = sum_b partial[b][k]
+// (one thread per bin; consecutive threads read consecutive bins of a block).
+template <typename R>
+__global__ void __launch_bounds__(256)
+    tapgrad_reduce_kernel(const cpx<R> *__restrict__ partial, int nblocks, int p,
+                          double2 *__restrict__ out)
+{
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= p) return;
+    double sx = 0., sy = 0.;
+    for (int b = 0; b < nblocks; ++b) {
+        const cpx<R> v = partial[(size_t)b * p + k];
+        sx += (double)v.x;
+        sy += (double)v.y;
+    }
+    out[k] = make_double2(sx, sy);
+}
+
 template <typename R, int P>
 static int launch_tapgrad_p(const TapGradLaunch &a, hipStream_t st)
 {
@@ -721,7 +740,17 @@ static int launch_tapgrad_t(const TapGradLaunch &a, hipStream_t st)
 
 int launch_tap_gradient(int real_bytes, const TapGradLaunch &a, hipStream_t st)
 {
-    return real_bytes == 8 ? launch_tapgrad_t<double>(a, st) : launch_tapgrad_t<float>(a, st);
+    int rc = real_bytes == 8 ? launch_tapgrad_t<double>(a, st) : launch_tapgrad_t<float>(a, st);
+    if (rc) return rc;
+    const dim3 grid((unsigned)((a.p + 255) / 256));
+    if (real_bytes == 8)
+        hipLaunchKernelGGL(tapgrad_reduce_kernel<double>, grid, dim3(256), 0, st,
+                           (const cpx<double> *)a.partial, a.nblocks, a.p, (double2 *)a.spectrum);
+    else
+        hipLaunchKernelGGL(tapgrad_reduce_kernel<float>, grid, dim3(256), 0, st,
+                           (const cpx<float> *)a.partial, a.nblocks, a.p, (double2 *)a.spectrum);
+    TKV_HIP(hipGetLastError());
+    return FDK_OK;
 }
 
 // ---------------------------------------------------------------- repack

@@ -185,4 +185,36 @@ void build_spectrum(int kind, const Geometry &g, const std::vector<double> &user
     }
 }
 
+void fft_pow2(std::vector<double> &re, std::vector<double> &im)
+{
+    const size_t n = re.size();
+    // bit-reversal permutation
+    for (size_t i = 1, j = 0; i < n; ++i) {
+        size_t bit = n >> 1;
+        for (; j & bit; bit >>= 1) j ^= bit;
+        j ^= bit;
+        if (i < j) {
+            std::swap(re[i], re[j]);
+            std::swap(im[i], im[j]);
+        }
+    }
+    for (size_t len = 2; len <= n; len <<= 1) {
+        const size_t half = len >> 1;
+        for (size_t k = 0; k < half; ++k) {
+            // e^{-2 pi i k / len}
+            const double a = -2.0 * M_PI * (double)k / (double)len;
+            const double wr = std::cos(a), wi = std::sin(a);
+            for (size_t s0 = 0; s0 < n; s0 += len) {
+                const size_t i0 = s0 + k, i1 = i0 + half;
+                const double xr = re[i1] * wr - im[i1] * wi;
+                const double xi = re[i1] * wi + im[i1] * wr;
+                re[i1] = re[i0] - xr;
+                im[i1] = im[i0] - xi;
+                re[i0] += xr;
+                im[i0] += xi;
+            }
+        }
+    }
+}
+
 }  // namespace tkv

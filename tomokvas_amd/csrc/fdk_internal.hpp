@@ -64,6 +64,9 @@ int slab_count(int Nz, int B);
 //   kind USER: H[k] = DFT_p(h)[k] complex for the caller's p spatial taps
 void build_spectrum(int kind, const Geometry &g, const std::vector<double> &user_taps,
                     std::vector<double> &Hre, std::vector<double> &Him);
+// in-place forward DFT X[j] = sum_k x[k] e^{-2 pi i k j / n} in double, n a
+// power of two (iterative radix-2; twiddles from the exact index)
+void fft_pow2(std::vector<double> &re, std::vector<double> &im);
 
 // ---------------------------------------------------------------- kernel launchers
 struct FilterLaunch {
@@ -124,6 +127,7 @@ struct TapGradLaunch {
     const void *tw;
     void *partial;          // [nblocks][p] complex Real
     int nblocks;
+    void *spectrum;         // [p] complex double: the sum of the partials
 };
 int launch_tap_gradient(int real_bytes, const TapGradLaunch &a, hipStream_t st);
 

@@ -11,7 +11,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import F16, F32, BF16, F64  # noqa: F401
+from ._lib import F16, F32, BF16, F64, F16_PACKED  # noqa: F401
 
 
 def _torch():
@@ -91,6 +91,21 @@ class Stages:
                                    nang, mp, store_dtype, stack.data_ptr(), scales.data_ptr(),
                                    _stream()), "fdk_pack")
         return stack, scales
+
+    def pack_into(self, filtered, store_dtype, stack, scales, ang0, maxabs=None):
+        """Packs filtered [Nv][n][Nu] into projections [ang0, ang0 + n) of a
+        stored stack of stack_bytes(store_dtype) bytes and its scales."""
+        _check_dev(filtered)
+        n = filtered.shape[1]
+        per = self.stack_bytes(store_dtype, 1) - self.stack_bytes(store_dtype, 0)
+        if ang0 < 0 or ang0 + n > scales.numel() or \
+                stack.numel() < self.stack_bytes(store_dtype, ang0 + n):
+            raise ValueError("pack_into: projections outside the stack")
+        mp = maxabs.data_ptr() if maxabs is not None else None
+        _lib.check(self.L.fdk_pack(self.fbp.handle, _dtype_code(filtered), filtered.data_ptr(),
+                                   n, mp, store_dtype, stack.data_ptr() + ang0 * per,
+                                   scales.data_ptr() + ang0 * scales.element_size(),
+                                   _stream()), "fdk_pack")
 
     def backproject(self, stack, store_dtype, scales=None, ang0=0, nang=None, vol=None,
                     accumulate=False):
