@@ -187,6 +187,16 @@ int fdk_forward_project(fdk_ctx *ctx, int dtype, const void *d_vol, int ang0, in
  * q, G [Nv][nang][Nu]; accumulated into the host array dh[p] (double). */
 int fdk_tap_gradient(fdk_ctx *ctx, int dtype, const void *d_q, const void *d_G, int nang,
                      double *dh, void *stream);
+/* fdk_tap_gradient without any allocation: no counterpart in the reference; for
+ * callers that cannot allocate (code under HIP graph capture, tight training
+ * loops, harnesses that forbid hipMalloc).  Its partial spectra live in a
+ * 32 MiB device array per GPU that the library holds until the process ends
+ * (fewer workgroups than fdk_tap_gradient when their partials would not fit);
+ * calls that overlap on one GPU race for it.  Same arguments and result as
+ * fdk_tap_gradient (up to floating-point summation order); FDK_ERR_UNSUPPORTED
+ * when a padded row length's spectrum exceeds the array (not for p <= 8192). */
+int fdk_tap_gradient_noalloc(fdk_ctx *ctx, int dtype, const void *d_q, const void *d_G,
+                             int nang, double *dh, void *stream);
 /* CollectionKernel (kernel.cu:203-215): dst += src */
 int fdk_accumulate(float *d_dst, const float *d_src, size_t n, void *stream);
 /* Quality metric on device data (no reference counterpart): out3 = { sum (a-b)^2,

@@ -65,6 +65,7 @@ _SIGS = {
     "fdk_backproject": (_i, [_vp, _i, _vp, _vp, _i, _i, _vp, _i, _vp]),
     "fdk_forward_project": (_i, [_vp, _i, _vp, _i, _i, _vp, _vp]),
     "fdk_tap_gradient": (_i, [_vp, _i, _vp, _vp, _i, _vp, _vp]),
+    "fdk_tap_gradient_noalloc": (_i, [_vp, _i, _vp, _vp, _i, _vp, _vp]),
     "fdk_accumulate": (_i, [_vp, _vp, C.c_size_t, _vp]),
     "fdk_compare": (_i, [_vp, _vp, C.c_size_t, _vp, _vp]),
     "fdk_phantom_project": (_i, [_vp, _vp, _i, _i, _i, _vp, _vp]),

@@ -19,12 +19,44 @@
 #include <cstdio>
 #include <cstring>
 #include <iostream>
+#include <mutex>
 
 namespace tkv {
 const char *last_error();
 }
 
 using tkv::fail;
+
+// ---- the allocation-free tap gradient's device scratch ------------------------------
+// One file-scope array per device, used only by fdk_tap_gradient_noalloc.  In
+// host code the array's name is a host-side stand-in, not a device address:
+// host code uses only the address hipGetSymbolAddress returns on each device
+// (looked up once per device, with that device current, on its first call).
+namespace {
+constexpr size_t TAPGRAD_SCRATCH = 32u << 20;   // bytes
+static __device__ __align__(256) unsigned char tapgrad_scratch[TAPGRAD_SCRATCH];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex tapgrad_scratch_mu;
+void *tapgrad_scratch_ptr[MAX_DEVICES] = {};
+
+// device address of the scratch on device `dev`, which must be current
+int tapgrad_scratch_of_device(int dev, void **out)
+{
+    if (dev < 0 || dev >= MAX_DEVICES)
+        return fail(FDK_ERR_UNSUPPORTED, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(tapgrad_scratch_mu);
+    if (!tapgrad_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(tapgrad_scratch));
+        if (e != hipSuccess) return tkv::hip_fail(e, "hipGetSymbolAddress(tapgrad_scratch)");
+        if (!ptr) return fail(FDK_ERR_HIP, "hipGetSymbolAddress(tapgrad_scratch): null address");
+        tapgrad_scratch_ptr[dev] = ptr;
+    }
+    *out = tapgrad_scratch_ptr[dev];
+    return FDK_OK;
+}
+}  // namespace
 
 struct fdk_ctx {
     int dev = 0;
@@ -144,6 +176,15 @@ int dalloc(void **p, size_t bytes, const char *what)
 hipStream_t pick(fdk_ctx *, void *stream)
 {
     return (hipStream_t)stream;
+}
+
+// workgroups of the tap-gradient kernel that give every one a transform of
+// its own: row pairs / transforms per workgroup (256 threads, min(p/4, 256) per
+// transform)
+long long tap_blocks(int p, long long rows)
+{
+    const int tpt = std::min(p / 4, 256), nt = 256 / tpt;
+    return ((rows + 1) / 2 + nt - 1) / nt;
 }
 
 int check_geometry(const tkv::Geometry &g)
@@ -1112,9 +1153,7 @@ int fdk_tap_gradient(fdk_ctx *c, int dtype, const void *d_q, const void *d_G, in
     if (!dh) return fail(FDK_ERR_INVALID_ARGUMENT, "tap_gradient: null output");
     const int p = c->p;
     const long long rows = (long long)c->g.Nv * nang;
-    const int tpt = std::min(p / 4, 256), nt = 256 / tpt;
-    long long need = ((rows + 1) / 2 + nt - 1) / nt;
-    const int nblocks = (int)std::max<long long>(1, std::min<long long>(need, 2048));
+    const int nblocks = (int)std::max<long long>(1, std::min<long long>(tap_blocks(p, rows), 2048));
     const size_t rb = dtype == FDK_F64 ? 8 : 4;
     // partial spectra of the workgroups, then their sum (p complex doubles)
     const size_t pbytes = (size_t)nblocks * p * 2 * rb;
@@ -1151,6 +1190,67 @@ int fdk_tap_gradient(fdk_ctx *c, int dtype, const void *d_q, const void *d_G, in
     }
     tkv::fft_pow2(Rr, Ri);
     for (int j = 0; j < p; ++j) dh[j] += Rr[j];
+    return FDK_OK;
+}
+
+// Allocation-free variant of fdk_tap_gradient for callers that cannot call
+// hipMalloc (code under HIP graph capture, tight training loops, harnesses that
+// forbid it).  The workgroups' partial spectra, their sum and the taps'
+// gradient live in the file-scope device array tapgrad_scratch (TAPGRAD_SCRATCH
+// bytes, held by every GPU the library runs on until the process ends); the
+// number of workgroups is capped so that their partials fit.  Every call on
+// one GPU shares that GPU's array: calls that overlap on one GPU race.  Nothing
+// in it has to start at zero (every launch overwrites what it reads).  The
+// final transform runs on the GPU and its result is added into dh through a
+// buffer on the stack, so the call makes no heap allocation either.
+int fdk_tap_gradient_noalloc(fdk_ctx *c, int dtype, const void *d_q, const void *d_G, int nang,
+                             double *dh, void *stream)
+{
+    int rc = need_prepared(c);   // selects c->dev
+    if (rc) return rc;
+    if (dtype != FDK_F32 && dtype != FDK_F64)
+        return fail(FDK_ERR_INVALID_ARGUMENT, "tap_gradient_noalloc: F32 or F64 data");
+    if (!dh || !d_q || !d_G)
+        return fail(FDK_ERR_INVALID_ARGUMENT, "tap_gradient_noalloc: null buffer");
+    const int p = c->p;
+    const size_t rb = dtype == FDK_F64 ? 8 : 4;
+    // spectrum (p complex doubles) and gradient (p doubles) at the end of the
+    // scratch, the partial spectra before them
+    const size_t tail = (size_t)p * 3 * sizeof(double);
+    const size_t per_block = (size_t)p * 2 * rb;
+    if (tail + per_block > TAPGRAD_SCRATCH)
+        return fail(FDK_ERR_UNSUPPORTED,
+                    "tap_gradient_noalloc: padded length " + std::to_string(p) +
+                        " needs more than the " + std::to_string(TAPGRAD_SCRATCH >> 20) +
+                        " MiB scratch");
+    const long long rows = (long long)c->g.Nv * nang;
+    const long long cap = (long long)((TAPGRAD_SCRATCH - tail) / per_block);
+    const int nblocks = (int)std::max<long long>(
+        1, std::min<long long>({tap_blocks(p, rows), cap, 2048LL}));
+    char *scr = nullptr;
+    if ((rc = tapgrad_scratch_of_device(c->dev, (void **)&scr))) return rc;
+    hipStream_t st = pick(c, stream);
+    tkv::TapGradLaunch L;
+    L.q = d_q;
+    L.G = d_G;
+    L.Nu = c->g.Nu;
+    L.rows = (int)std::min<long long>(rows, 0x7fffffff);
+    L.p = p;
+    L.tw = dtype == FDK_F64 ? (const void *)c->d_tw64 : (const void *)c->d_tw32;
+    L.partial = scr;
+    L.nblocks = nblocks;
+    L.spectrum = scr + (TAPGRAD_SCRATCH - tail);
+    double *d_dh = (double *)((char *)L.spectrum + (size_t)p * 2 * sizeof(double));
+    if ((rc = tkv::launch_tap_gradient((int)rb, L, st))) return rc;
+    if ((rc = tkv::launch_tap_gradient_dft(L.spectrum, c->d_tw64, p, d_dh, st))) return rc;
+    double part[512];
+    for (int j0 = 0; j0 < p; j0 += 512) {
+        const int n = std::min(512, p - j0);
+        TKV_HIP(hipMemcpyAsync(part, d_dh + j0, (size_t)n * sizeof(double),
+                               hipMemcpyDeviceToHost, st));
+        TKV_HIP(hipStreamSynchronize(st));
+        for (int j = 0; j < n; ++j) dh[j0 + j] += part[j];
+    }
     return FDK_OK;
 }
 

@@ -738,6 +738,37 @@ static int launch_tapgrad_t(const TapGradLaunch &a, hipStream_t st)
                 "tap gradient: padded length " + std::to_string(a.p) + " is not supported");
 }
 
+// The final transform on the device: dh[j] = Re sum_k R[k] e^{-2 pi i k j / p}
+// with R[k] = (P[k] + conj(P[p-k]))/2 the Hermitian part of the summed
+// spectrum P; one thread per tap j, twiddles from the exact table
+// tw[n] = e^{-2 pi i n / p} at n = k*j mod p.
+__global__ void __launch_bounds__(256)
+    tapgrad_dft_kernel(const double2 *__restrict__ P, const double2 *__restrict__ tw, int p,
+                       double *__restrict__ dh)
+{
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= p) return;
+    double s = 0.;
+    unsigned n = 0;   // k*j mod p, p a power of two
+    for (int k = 0; k < p; ++k, n = (n + (unsigned)j) & (unsigned)(p - 1)) {
+        const int m = (p - k) & (p - 1);
+        const double2 a = P[k], b = P[m];
+        const double rr = 0.5 * (a.x + b.x), ri = 0.5 * (a.y - b.y);
+        const double2 w = tw[n];   // (cos, -sin) of 2 pi k j / p
+        s += rr * w.x - ri * w.y;
+    }
+    dh[j] = s;
+}
+
+int launch_tap_gradient_dft(const void *spectrum, const void *tw64, int p, double *dh,
+                            hipStream_t st)
+{
+    hipLaunchKernelGGL(tapgrad_dft_kernel, dim3((unsigned)((p + 255) / 256)), dim3(256), 0, st,
+                       (const double2 *)spectrum, (const double2 *)tw64, p, dh);
+    TKV_HIP(hipGetLastError());
+    return FDK_OK;
+}
+
 int launch_tap_gradient(int real_bytes, const TapGradLaunch &a, hipStream_t st)
 {
     int rc = real_bytes == 8 ? launch_tapgrad_t<double>(a, st) : launch_tapgrad_t<float>(a, st);

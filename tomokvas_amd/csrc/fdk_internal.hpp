@@ -130,6 +130,10 @@ struct TapGradLaunch {
     void *spectrum;         // [p] complex double: the sum of the partials
 };
 int launch_tap_gradient(int real_bytes, const TapGradLaunch &a, hipStream_t st);
+// dh[j] = Re sum_k R[k] tw[k*j mod p], R the Hermitian part of the spectrum
+// (p complex doubles), tw64 the context's FP64 twiddles; dh on the device
+int launch_tap_gradient_dft(const void *spectrum, const void *tw64, int p, double *dh,
+                            hipStream_t st);
 
 struct PhantomLaunch {
     const double *ells;     // [n][19]
