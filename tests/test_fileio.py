@@ -214,3 +214,30 @@ def test_bin_rejects_bad_slabs(tmp_path):
     with pytest.raises(RuntimeError):
         b.SetScanBlockOffset(1)
     b.close()
+
+
+def test_make_cra_header_and_raw_read_back(tmp_path):
+    # tomokvas_amd.make_cra writes a workload's scan; its host part against the
+    # reader: the header gives back the geometry and the raw file the stack
+    from tomokvas_amd import geometry
+    from tomokvas_amd.make_cra import write_cra_files
+    # sizes inside the reader's clamps (x <= detector width, z below the cone limit)
+    g = geometry.ScanGeometry(Nu=48, Nv=40, Nang=30, tau=0.01, D=400.0, Nx=36, Ny=36, Nz=24,
+                              scale=1.25)
+    stack = coded_raw(1, g.Nang, g.Nv, g.Nu).transpose(1, 0, 2).copy()    # [v][beta][u]
+    path = write_cra_files(str(tmp_path), "syn", g, stack, "synthetic test")
+    assert path == str(tmp_path / "syn.CRA")
+    cra = CraCon(str(tmp_path), "syn")
+    assert cra.GetMetaData() == _lib.FDK_OK
+    m = cra.meta
+    assert (m.detector_x, m.detector_z, m.num_of_projections) == (g.Nu, g.Nv, g.Nang)
+    assert (m.x_dimension, m.z_dimension) == (45, 30)            # N * scale
+    assert m.sample_spacing == pytest.approx(g.tau * 1e4)
+    assert cra.GetDistance() == pytest.approx(g.D, rel=1e-6)
+    assert cra.GetScale() == pytest.approx(1 / g.scale)
+    assert cra.GetDFI() == pytest.approx(2 * math.pi / g.Nang, rel=1e-6)
+    assert cra.GetCentreOfRotation() == pytest.approx(g.cX)
+    cra.SetSinBlockMaxSize(1 << 30)
+    assert cra.CalculateSinBlocks() == 1
+    cra.SetSinActiveBlock(0)
+    assert np.array_equal(cra.LoadData(), stack)

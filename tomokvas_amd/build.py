@@ -34,6 +34,7 @@ SOURCES = [
     "fdk_driver.cpp",
     "fdk_rsq.hip",
     "fdk_tiff.hip",
+    "fdk_tiff_host.cpp",
 ]
 # the command line program, linked against the library (host code only)
 MAIN = "fdk_main.cpp"
