@@ -230,14 +230,20 @@ struct Plan {
     static constexpr int RLAST = radix(NPASS - 1);
 };
 
-// LDS index with one pad slot per 16 values (bank-conflict relief), when the
-// padded transforms still fit 64 KiB per workgroup
+// LDS index with one pad slot per 2^TKV_FILTER_PAD_SHIFT values (bank-conflict
+// relief; 0: no padding), when the padded transforms still fit 64 KiB per
+// workgroup
+#ifndef TKV_FILTER_PAD_SHIFT
+#define TKV_FILTER_PAD_SHIFT 4
+#endif
 template <typename R, typename PL>
 struct Pad {
     static constexpr int P = PL::P;
-    static constexpr bool ON = sizeof(cpx<R>) * (size_t)PL::NT * (P + P / 16) <= 65536;
-    static constexpr int SIZE = ON ? P + P / 16 : P;
-    static __device__ __forceinline__ int at(int n) { return ON ? n + (n >> 4) : n; }
+    static constexpr int SH = TKV_FILTER_PAD_SHIFT;
+    static constexpr bool ON =
+        SH > 0 && sizeof(cpx<R>) * (size_t)PL::NT * (P + (P >> SH)) <= 65536;
+    static constexpr int SIZE = ON ? P + (P >> SH) : P;
+    static __device__ __forceinline__ int at(int n) { return ON ? n + (n >> SH) : n; }
 };
 
 // forward twiddle W_N^q = exp(-2 pi i q / N), N in {8, 16}: angles are
