@@ -6,8 +6,10 @@ Splitting the angles changes only the FP32 summation order and the float
 rounding of each block's start angle (|d rot| <= 2.4e-7 rad here, a shift of
 < 2e-5 px at the volume's edge); the projections are smooth, so the files agree
 with the one-pass volume to 1e-5 of its largest value.  With FP16 storage the
-interpolation weights are rounded to FP16 (steps of 2^-11 below 1), and such a
-shift can move a weight across a rounding boundary: 1e-4 there.
+filtered samples are rounded to FP16 after filtering, and the filter pairs
+detector rows differently when a block holds fewer angles (two real rows per
+complex transform), so a sample that lies next to an FP16 rounding boundary can
+round to the other neighbour (2^-11 of it): 1e-4 there.
 """
 import math
 import os
