@@ -4,6 +4,11 @@ from __future__ import annotations
 import sys
 
 import numpy as np
+# torch ships its own HIP runtime; it has to be the first one loaded into the
+# process (the library then binds to the same one), else torch finds no device.
+# Every GPU test module imports this helper, so running one file alone keeps
+# the order the full suite gets from its collection.
+import torch  # noqa: F401
 
 import oracle as O
 from tomokvas_amd import _lib, geometry
