@@ -331,9 +331,11 @@ def fdk_steps(fbp, L, hipm, steps, warmup):
     return [float(np.mean(s[:, i])) for i in range(3)], 1e3 * float(np.mean(walls))
 
 
-def config_leg(name, hipm, storages, steps):
+def config_leg(name, hipm, storages, steps, cpu_full=False, threads=None):
     """GUPS and FDK seconds of one workload per storage type, with each
-    low-precision volume's relative RMSE against the FP32 volume (on the GPU)."""
+    low-precision volume's relative RMSE against the FP32 volume (on the GPU).
+    cpu_full: also reconstruct the workload in full with the CPU FP32 mirror
+    (one timed run) and compare the GPU FP32 volume with it."""
     L = _lib.load()
     w = geometry.workloads()[name]
     g = w.geom
@@ -359,17 +361,34 @@ def config_leg(name, hipm, storages, steps):
         e = None
         if st == _lib.F32:
             assert hipm.hipMemcpy(ref, dvol, nvol * 4, 3) == 0
-        else:
+        elif _lib.F32 in storages:       # FP32 ran first: compare with its volume
             o3 = (C.c_double * 3)()
             _lib.check(L.fdk_compare(dvol, ref, nvol, o3, None), "compare")
             e = math.sqrt(o3[0] / o3[1])
         out[NAMES[st]] = {"gups": g.voxel_updates / (bp * 1e-3) / 1e9, "fdk_s": (f + pk + bp) / 1e3,
                           "filter_ms": f, "pack_ms": pk, "bp_ms": bp,
                           "rel_rmse_vs_f32_volume": e}
+    if cpu_full and _lib.F32 in storages:
+        import oracle as O
+        raw = np.empty((g.Nv, g.Nang, g.Nu), np.float32)
+        assert hipm.hipMemcpy(raw.ctypes.data, C.c_void_p(fbp.device_sinogram_ptr()),
+                              raw.nbytes, 2) == 0
+        gpu = np.empty((g.Nz, g.Ny, g.Nx), np.float32)
+        assert hipm.hipMemcpy(gpu.ctypes.data, ref, nvol * 4, 2) == 0
+        t0 = time.perf_counter()
+        mirror = O.fdk(O.Geometry(**g.oracle_dict()), raw, kind=w.filter)
+        t = time.perf_counter() - t0
+        out["cpu_full"] = {"fdk_seconds": t, "gups": g.voxel_updates / t / 1e9,
+                           "threads": threads, "kind": "port",
+                           "gpu_f32_vs_cpu_mirror_rel_rmse": O.rel_rmse(gpu, mirror)}
+        log(f"{name} in full on the CPU ({threads} threads): {t:.1f} s "
+            f"({out['cpu_full']['gups']:.2f} GUPS), GPU FP32 vs mirror "
+            f"{out['cpu_full']['gpu_f32_vs_cpu_mirror_rel_rmse']:.1e}")
+        del raw, gpu, mirror
     hipm.hipFree(ref)
     fbp.close()
     log(f"{name}: " + ", ".join(f"{k} {v['fdk_s'] * 1e3:.2f} ms ({v['gups']:.0f} GUPS)"
-                                for k, v in out.items() if isinstance(v, dict)))
+                                for k, v in out.items() if isinstance(v, dict) and "fdk_s" in v))
     return out
 
 
@@ -655,8 +674,15 @@ def main():
                               "read_bytes": 8 * n, "write_bytes": 4 * n}
     fbp.close()
     if not args.no_configs:
-        out["configs"] = {name: config_leg(name, hipm, order, min(args.steps, 3))
+        cpu2 = out["cpu_baseline"] is not None
+        out["configs"] = {name: config_leg(name, hipm, order, min(args.steps, 3),
+                                           cpu_full=cpu2 and name == "cfg2", threads=args.threads)
                           for name in ("cfg1", "cfg2", "cfg4", "cfg5") if name != args.workload}
+        c2 = out["configs"].get("cfg2", {}).get("cpu_full")
+        if c2:
+            # the CPU baseline's second full reconstruction: 512^3 from 900 x 1024^2
+            out["cpu_baseline"]["cfg2_full_fdk_seconds"] = c2["fdk_seconds"]
+            out["cpu_baseline"]["cfg2_gups"] = c2["gups"]
     if not args.no_diff:
         # the training steps run on torch tensors: in a child process that
         # initialises torch before anything else touches the GPU
