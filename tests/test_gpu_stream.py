@@ -35,7 +35,7 @@ def d2h(ptr, shape):
 
 def one_launch(g, s, storage):
     """No sink: the whole volume backprojected (in angle chunks of >= 100
-    angles, at most 8), read from HBM."""
+    angles, at most 16), read from HBM."""
     fbp = engine(g, s, None, storage=storage)
     _lib.check(fbp.Initialize(), "Initialize")
     _lib.check(fbp.CalculateReconstruction(), "CalculateReconstruction")
@@ -82,6 +82,20 @@ def test_chunked_slabs_equal_one_launch(Nz, B, storage):
         np.testing.assert_array_equal(sink.volume, want)
         say(f"  chunks of <= {chunk} slices; BP {times[2]:.3f} ms, "
             f"exposed transfers {times[3]:.3f} ms")
+
+
+@pytest.mark.parametrize("storage", [_lib.F32, _lib.F16])
+def test_handout_angle_split_equals_one_pass(storage):
+    # 700 projections: 7 angle chunks of 100, the first over the whole volume,
+    # the last 6 through the slab pipeline in z-chunks -- the same launches
+    # per voxel, in the same order, as the pass without a sink
+    g = scan(32, 24, 700, 32, 32, 70, B=8)
+    s = np.random.default_rng(7).standard_normal((g.Nv, g.Nang, g.Nu)).astype(np.float32)
+    want = one_launch(g, s, storage)
+    sink, chunk, times, _ = piped(g, s, storage, _lib.VOLUME_RESIDENT)
+    say(f"  hand-out over 7 angle chunks: z-chunks of <= {chunk} slices, BP {times[2]:.3f} ms")
+    assert [c[0] for c in sink.calls] == list(range(-(-g.Nz // g.B)))
+    np.testing.assert_array_equal(sink.volume, want)
 
 
 def test_streamed_collecting_two_angle_blocks():

@@ -438,14 +438,34 @@ int backproject_in_angle_chunks(fdk_ctx *c, const std::vector<int> &ab, void *vo
     return FDK_OK;
 }
 
-// Largest chunk for the current storage type: half the volume when it is
-// resident; an eighth when it is streamed, within 2 C + B <= ring_slices (see
-// backproject_and_hand_out).
+// The slab hand-out of a resident volume: the angles in TKV_HANDOUT_ANGLE_CHUNKS
+// chunks (those of a whole-volume pass, so both give the same volume), all but
+// the last TKV_HANDOUT_PIPED_CHUNKS over the whole volume, those through the
+// slab pipeline in z-chunks of at most 1/TKV_HANDOUT_ZPARTS of the volume (see
+// backproject_and_hand_out).  The piped angles must last longer than the slab
+// transfers (4.3 GB at 57 GB/s for 1024^3), and the last z-chunks' transfers
+// are the ones left exposed.  Measured at 1024^3 from 1800 projections, FP16,
+// page-locked slabs (FDK wall time; 559 ms without hand-out): 8 chunks with the
+// last piped in halves 610-629 ms; 16 with the last 4 in 1/8 parts 581, in 1/4
+// parts 589, in 1/16 parts 601; the last 6 in 1/8 parts 577; the last 8, 582.
+#ifndef TKV_HANDOUT_ANGLE_CHUNKS
+#define TKV_HANDOUT_ANGLE_CHUNKS TKV_BP_ANGLE_CHUNKS
+#endif
+#ifndef TKV_HANDOUT_PIPED_CHUNKS
+#define TKV_HANDOUT_PIPED_CHUNKS 6
+#endif
+#ifndef TKV_HANDOUT_ZPARTS
+#define TKV_HANDOUT_ZPARTS 8
+#endif
+
+// Largest chunk for the current storage type: 1/TKV_HANDOUT_ZPARTS of the
+// volume when it is resident; an eighth when it is streamed, within
+// 2 C + B <= ring_slices (see backproject_and_hand_out).
 int pipeline_chunk(const fdk_ctx *c)
 {
     const tkv::Geometry &g = c->g;
     const int zr = tkv::bp_zrun(c->storage);
-    if (c->volume_mode != FDK_VOLUME_STREAMED) return chunk_target(g, zr, 2);
+    if (c->volume_mode != FDK_VOLUME_STREAMED) return chunk_target(g, zr, TKV_HANDOUT_ZPARTS);
     int C = chunk_target(g, zr, 8);
     if (c->ring_slices < g.Nz) C = std::min(C, (c->ring_slices - g.B) / 2 / zr * zr);
     return C;
@@ -466,10 +486,12 @@ int pipeline_chunk(const fdk_ctx *c)
 // Records ev[3] after the last backprojection and ev[4] after the last copy,
 // so ev[3] -> ev[4] is the transfer time left exposed.
 // Angles [ang0, ang0 + nang) of the stored stack (stack and scales point at
-// angle ang0), added onto the volume when accumulate is set.
+// angle ang0), added onto the volume when accumulate is set; a resident
+// volume's chunks are backprojected in the angle launches `sub` (bounds
+// relative to ang0, from 0 to nang).
 int backproject_and_hand_out(fdk_ctx *c, fdk_get_block_fn get_block, void *user,
                              const void *stack, const float *scales, int ang0, int nang,
-                             int accumulate)
+                             int accumulate, const std::vector<int> &sub_resident)
 {
     const tkv::Geometry &g = c->g;
     hipStream_t st = c->stream, cs = c->copy_stream;
@@ -493,7 +515,9 @@ int backproject_and_hand_out(fdk_ctx *c, fdk_get_block_fn get_block, void *user,
     // a streamed volume's chunks also cut their angles (the first launch
     // writes the chunk's slot, the later ones add onto it)
     const std::vector<int> sub =
-        streamed ? angle_bounds(nang, TKV_STREAM_ANGLE_CHUNKS) : std::vector<int>{0, nang};
+        streamed ? angle_bounds(nang, TKV_STREAM_ANGLE_CHUNKS) : sub_resident;
+    if (sub.size() < 2 || sub.front() != 0 || sub.back() != nang)
+        return fail(FDK_ERR_INVALID_ARGUMENT, "slab pipeline: angle launches do not cover the angles");
     const size_t stored = (size_t)sample_bytes(c->storage) * tkv::stack_pitch(g.Nu) * g.Nv;
     auto bp = [&](int k) -> int {
         if (wraps && k >= 2) TKV_HIP(hipStreamWaitEvent(st, c->ev_cp[(k - 2) & 1], 0));
@@ -905,24 +929,27 @@ int fdk_calculate_reconstruction(fdk_ctx *c, fdk_get_block_fn get_block, void *u
         TKV_HIP(hipEventRecord(c->ev[3], st));
         TKV_HIP(hipEventRecord(c->ev[4], st));
     } else {
-        // resident: all angle chunks but the last over the whole volume, the
-        // last through the slab pipeline (slabs leave as its z-chunks finish;
-        // at most 8 chunks, so that the last one lasts long enough to hide
-        // most of the slab transfers);
+        // resident: all angle chunks but the last TKV_HANDOUT_PIPED_CHUNKS
+        // over the whole volume, those through the slab pipeline (slabs leave
+        // as its z-chunks finish, so the piped angles must last long enough
+        // to hide most of the slab transfers);
         // streamed: one chunk of every angle through the pipeline
-        const std::vector<int> ab = c->volume_mode == FDK_VOLUME_STREAMED
-                                        ? std::vector<int>{0, g.Nang}
-                                        : angle_bounds(g.Nang, 8);
+        const bool streamed = c->volume_mode == FDK_VOLUME_STREAMED;
+        const std::vector<int> ab = streamed ? std::vector<int>{0, g.Nang}
+                                             : angle_bounds(g.Nang, TKV_HANDOUT_ANGLE_CHUNKS);
         const int K = (int)ab.size() - 1;
+        const int Pc = streamed ? 1 : std::min(K, TKV_HANDOUT_PIPED_CHUNKS);
         const size_t stored =
             (size_t)sample_bytes(c->storage) * tkv::stack_pitch(g.Nu) * g.Nv;
-        if (K > 1 && (rc = backproject_in_angle_chunks(
-                          c, std::vector<int>(ab.begin(), ab.end() - 1), c->d_vol, 0, st)))
+        if (K > Pc && (rc = backproject_in_angle_chunks(
+                           c, std::vector<int>(ab.begin(), ab.end() - Pc), c->d_vol, 0, st)))
             return rc;
-        const int a0 = ab[K - 1];
+        const int a0 = ab[K - Pc];
+        std::vector<int> sub;
+        for (int k = K - Pc; k <= K; ++k) sub.push_back(ab[k] - a0);
         if ((rc = backproject_and_hand_out(c, get_block, user,
                                            (const char *)c->d_stack + (size_t)a0 * stored,
-                                           c->d_scales + a0, a0, g.Nang - a0, K > 1))) {
+                                           c->d_scales + a0, a0, g.Nang - a0, K > Pc, sub))) {
             (void)hipStreamSynchronize(c->copy_stream);
             return rc;
         }
@@ -1001,7 +1028,7 @@ int fdk_reconstruct_host(fdk_ctx *c, const float *host_sino, fdk_get_block_fn ge
                 TKV_HIP(hipEventRecord(c->ev[4], st));
             }
         } else if ((rc = backproject_and_hand_out(c, get_block, user, stack, scales, a0, nc,
-                                                  K > 1))) {
+                                                  K > 1, std::vector<int>{0, nc}))) {
             break;
         }
     }
