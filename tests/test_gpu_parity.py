@@ -83,6 +83,20 @@ def test_engine_odd_geometry_and_dropped_slab():
     assert O.rel_rmse(vol[16], ref[16]) <= TOL_FDK
 
 
+def test_engine_wide_detector():
+    # 5000 detector columns: p = 16384, one row transform per workgroup of 1024
+    # threads in 128 KiB of LDS
+    g = scan(5000, 8, 6, 24, 24, 1, D=6000.0, scale=150.0)
+    assert g.p == 16384
+    sino = np.random.default_rng(5).standard_normal((g.Nv, g.Nang, g.Nu)).astype(np.float32)
+    vol, _, _ = run_engine(g, sino)
+    ref = O.fdk(ogeom(g), sino, kind=0)
+    assert np.abs(ref).max() > 0
+    e = O.rel_rmse(vol, ref)
+    say(f"  Nu = 5000: FP32 vs mirror {e:.2e}")
+    assert e <= TOL_FDK
+
+
 # degenerate shapes: one projection, one detector row, one slice, one voxel,
 # a one-column detector (p = 2)
 DEGENERATE = {
@@ -188,7 +202,7 @@ def test_weight_kernel_matches_mirror(torch):
 
 
 @pytest.mark.parametrize("Nu", [1, 2, 3, 5, 9, 17, 37, 64, 100, 200, 300, 600, 1024, 1536, 2048,
-                                3000])
+                                3000, 4096, 4097, 6000, 8192])
 def test_filter_kernel_all_lengths(torch, Nu):
     g = scan(Nu, 3, 5, 8, 8, 8, D=max(2500.0, Nu))
     for filt in (0, 1):
@@ -205,7 +219,7 @@ def test_filter_kernel_all_lengths(torch, Nu):
         assert e <= 2e-6, (Nu, filt, e)
         m = maxabs.cpu().numpy().view(np.float32)
         assert np.allclose(m, np.abs(y).max(axis=(0, 2)), rtol=0, atol=0)
-        if g.p <= 4096:
+        if g.p <= 8192:   # FP64 rows up to 4096 samples (p = 8192: 128 KiB of LDS)
             d64 = torch.from_numpy(x).cuda()
             st.filter(d64)
             assert O.rel_rmse(d64.cpu().numpy(), ref) <= 1e-13, (Nu, filt)

@@ -198,8 +198,8 @@ int check_geometry(const tkv::Geometry &g)
     if (!(g.D > 0.5f * g.Nu))
         return fail(FDK_ERR_INVALID_ARGUMENT,
                     "source distance must exceed Nu/2 (the field-of-view radius)");
-    if (tkv::padded_length(g.Nu) > 8192)
-        return fail(FDK_ERR_UNSUPPORTED, "detector rows longer than 4096 are not supported");
+    if (tkv::padded_length(g.Nu) > 16384)
+        return fail(FDK_ERR_UNSUPPORTED, "detector rows longer than 8192 are not supported");
     if (!(g.scale > 0)) return fail(FDK_ERR_INVALID_ARGUMENT, "scale must be positive");
     return FDK_OK;
 }

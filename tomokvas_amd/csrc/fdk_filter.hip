@@ -566,7 +566,12 @@ static int launch_filter_plan(const FilterLaunch &a, hipStream_t st)
                      ? sizeof(cpx<R>) * (size_t)PL::NT * Pad<R, PL>::SIZE
                      : 0;
     auto kern = filter_kernel<R, PL, WAVES>;
-    if (lds > 65536) return fail(FDK_ERR_UNSUPPORTED, "filter: transform does not fit LDS");
+    // beyond 64 KiB (one transform of p = 16384 in FP32, 8192 in FP64) the
+    // workgroup takes up to the 160 KiB of LDS of a gfx950 CU
+    if (lds > 163840) return fail(FDK_ERR_UNSUPPORTED, "filter: transform does not fit LDS");
+    if (lds > 65536)
+        TKV_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)lds));
     hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(PL::BLOCK), lds, st, (const R *)a.in,
                        (R *)a.out, a.Nu, a.Nv, a.nang, a.weight, (R)a.cX, (R)a.cZ, (R)a.D,
                        (const R *)a.Hre, (const R *)a.Him, a.conj, (const cpx<R> *)a.tw,
@@ -575,12 +580,18 @@ static int launch_filter_plan(const FilterLaunch &a, hipStream_t st)
     return FDK_OK;
 }
 
-// long rows (p >= 2048) in FP32: radix-8 passes over 512 threads per
-// workgroup, 8 values per thread; otherwise radix 16 over 256 threads
+// radix 16 over 256 threads; the longest rows (FP32 p = 16384, FP64 p = 8192)
+// one transform per workgroup of 1024 / 512 threads, 16 values per thread (the
+// transform alone fills 128 KiB of LDS); TKV_FILTER_R8: long FP32 rows
+// (p >= 2048) in radix-8 passes over 512 threads, 8 values per thread
 template <typename R, int P>
 static int launch_filter_p(const FilterLaunch &a, hipStream_t st)
 {
-    if constexpr (TKV_FILTER_R8 && sizeof(R) == 4 && P >= 2048)
+    if constexpr (sizeof(R) == 4 && P > 8192)
+        return launch_filter_plan<R, Plan<P, 16, 1024>, TKV_FILTER_WAVES>(a, st);
+    else if constexpr (sizeof(R) == 8 && P > 4096)
+        return launch_filter_plan<R, Plan<P, 16, 512>, 2>(a, st);
+    else if constexpr (TKV_FILTER_R8 && sizeof(R) == 4 && P >= 2048)
         return launch_filter_plan<R, Plan<P, 8, 512>, TKV_FILTER_R8_WAVES>(a, st);
     else
         // FP64 values take twice the registers: compiled for 3 waves
@@ -602,8 +613,9 @@ static int launch_filter_t(const FilterLaunch &a, hipStream_t st)
     case 1024: return launch_filter_p<R, 1024>(a, st);
     case 2048: return launch_filter_p<R, 2048>(a, st);
     case 4096: return launch_filter_p<R, 4096>(a, st);
-    case 8192:
-        if constexpr (sizeof(R) == 4) return launch_filter_p<R, 8192>(a, st);
+    case 8192: return launch_filter_p<R, 8192>(a, st);
+    case 16384:
+        if constexpr (sizeof(R) == 4) return launch_filter_p<R, 16384>(a, st);
         break;
     default: break;
     }
