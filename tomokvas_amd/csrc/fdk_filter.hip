@@ -699,22 +699,37 @@ __global__ void __launch_bounds__(256)
     }
 }
 
-// Sum of the workgroups' partial spectra, in double: out[k] = sum_b partial[b][k]
-// (one thread per bin; consecutive threads read consecutive bins of a block).
+// Sum of the workgroups' partial spectra, in double: out[k] = sum_b partial[b][k].
+// A workgroup owns 64 consecutive bins (one per lane, so each block's row is
+// read in 64-bin segments); its 16 waves take the blocks b = w, w + 16, ...
+// and their 16 sums are added in a fixed tree, so the result does not depend
+// on scheduling.
 template <typename R>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(1024)
     tapgrad_reduce_kernel(const cpx<R> *__restrict__ partial, int nblocks, int p,
                           double2 *__restrict__ out)
 {
-    const int k = blockIdx.x * 256 + threadIdx.x;
-    if (k >= p) return;
+    __shared__ double2 part[16][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int k = blockIdx.x * 64 + lane;
     double sx = 0., sy = 0.;
-    for (int b = 0; b < nblocks; ++b) {
-        const cpx<R> v = partial[(size_t)b * p + k];
-        sx += (double)v.x;
-        sy += (double)v.y;
+    if (k < p)
+        for (int b = w; b < nblocks; b += 16) {
+            const cpx<R> v = partial[(size_t)b * p + k];
+            sx += (double)v.x;
+            sy += (double)v.y;
+        }
+    part[w][lane] = make_double2(sx, sy);
+    __syncthreads();
+#pragma unroll
+    for (int h = 8; h > 0; h >>= 1) {
+        if (w < h) {
+            const double2 o = part[w + h][lane];
+            part[w][lane] = make_double2(part[w][lane].x + o.x, part[w][lane].y + o.y);
+        }
+        __syncthreads();
     }
-    out[k] = make_double2(sx, sy);
+    if (w == 0 && k < p) out[k] = part[0][lane];
 }
 
 template <typename R, int P>
@@ -791,12 +806,12 @@ int launch_tap_gradient(int real_bytes, const TapGradLaunch &a, hipStream_t st)
 {
     int rc = real_bytes == 8 ? launch_tapgrad_t<double>(a, st) : launch_tapgrad_t<float>(a, st);
     if (rc) return rc;
-    const dim3 grid((unsigned)((a.p + 255) / 256));
+    const dim3 grid((unsigned)((a.p + 63) / 64));
     if (real_bytes == 8)
-        hipLaunchKernelGGL(tapgrad_reduce_kernel<double>, grid, dim3(256), 0, st,
+        hipLaunchKernelGGL(tapgrad_reduce_kernel<double>, grid, dim3(1024), 0, st,
                            (const cpx<double> *)a.partial, a.nblocks, a.p, (double2 *)a.spectrum);
     else
-        hipLaunchKernelGGL(tapgrad_reduce_kernel<float>, grid, dim3(256), 0, st,
+        hipLaunchKernelGGL(tapgrad_reduce_kernel<float>, grid, dim3(1024), 0, st,
                            (const cpx<float> *)a.partial, a.nblocks, a.p, (double2 *)a.spectrum);
     TKV_HIP(hipGetLastError());
     return FDK_OK;
