@@ -168,8 +168,8 @@ int fdk_filter(fdk_ctx *ctx, int dtype, void *d_sino, int nang, int weight, int 
  * [nang][Nv][pitch] samples, pitch >= Nu + 68 (zero-padded rows) */
 size_t fdk_stack_bytes(fdk_ctx *ctx, int store_dtype, int nang);
 /* repack filtered [Nv][nang][Nu] (F32 or F64) into the backprojector's stored
- * stack; FP16 stores value/scale with scale = maxabs/8192 per projection
- * (d_maxabs required), F32/BF16/F64 store the values (scale 1) */
+ * stack (16-byte aligned); FP16 stores value/scale with scale = maxabs/8192 per
+ * projection (d_maxabs required), F32/BF16/F64 store the values (scale 1) */
 int fdk_pack(fdk_ctx *ctx, int dtype_in, const void *d_filtered, int nang,
              const uint32_t *d_maxabs, int store_dtype, void *d_stack, void *d_scales,
              void *stream);

@@ -1098,6 +1098,8 @@ int fdk_pack(fdk_ctx *c, int dtype_in, const void *d_filtered, int nang,
     if (rc) return rc;
     if (!d_filtered || !d_stack || !d_scales)
         return fail(FDK_ERR_INVALID_ARGUMENT, "pack: null buffer");
+    if ((uintptr_t)d_stack & 15)
+        return fail(FDK_ERR_INVALID_ARGUMENT, "pack: stored stack not 16-byte aligned");
     return tkv::launch_pack(dtype_in, d_filtered, c->g.Nu, c->g.Nv, nang, tkv::stack_pitch(c->g.Nu),
                             d_maxabs, store_dtype, d_stack, d_scales, pick(c, stream));
 }
