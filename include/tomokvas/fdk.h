@@ -174,8 +174,9 @@ int fdk_pack(fdk_ctx *ctx, int dtype_in, const void *d_filtered, int nang,
              const uint32_t *d_maxabs, int store_dtype, void *d_stack, void *d_scales,
              void *stream);
 /* FBPConeKernel (kernel.cu:68-177) over the whole volume for angles
- * [ang0, ang0+nang): vol (+)= delta_f/p * sum U^2 * bilinear(Q).  vol is F64 for
- * F64 storage, F32 otherwise. */
+ * [ang0, ang0+nang): vol (+)= delta_f/p * sum U^2 * bilinear(Q).  d_stack and
+ * d_scales hold those nang projections (d_scales NULL: all 1).  vol is F64 for
+ * F64 storage, F32 otherwise.  Runs in the angle chunks of a whole FDK pass. */
 int fdk_backproject(fdk_ctx *ctx, int store_dtype, const void *d_stack,
                     const void *d_scales, int ang0, int nang, void *d_vol, int accumulate,
                     void *stream);

@@ -1111,8 +1111,22 @@ int fdk_backproject(fdk_ctx *c, int store_dtype, const void *d_stack, const void
     if (rc) return rc;
     if ((rc = check_angles(c, ang0, nang))) return rc;
     if (!d_stack || !d_vol) return fail(FDK_ERR_INVALID_ARGUMENT, "backproject: null buffer");
-    return stage_backproject(c, store_dtype, d_stack, d_scales, ang0, nang, d_vol, accumulate,
-                             pick(c, stream));
+    if (!sample_bytes(store_dtype))
+        return fail(FDK_ERR_INVALID_ARGUMENT, "backproject: unknown storage type");
+    // in the angle chunks of a whole pass (launch boundaries keep the
+    // workgroups' projections together in L2, see angle_bounds)
+    hipStream_t st = pick(c, stream);
+    const std::vector<int> ab = angle_bounds(nang);
+    const size_t stored = (size_t)sample_bytes(store_dtype) * tkv::stack_pitch(c->g.Nu) * c->g.Nv;
+    const size_t sb = store_dtype == FDK_F64 ? sizeof(double) : sizeof(float);
+    for (size_t k = 0; k + 1 < ab.size(); ++k) {
+        const int a = ab[k], n = ab[k + 1] - a;
+        const void *sc = d_scales ? (const void *)((const char *)d_scales + a * sb) : nullptr;
+        if ((rc = stage_backproject(c, store_dtype, (const char *)d_stack + a * stored, sc,
+                                    ang0 + a, n, d_vol, accumulate || k > 0, st)))
+            return rc;
+    }
+    return FDK_OK;
 }
 
 int fdk_forward_project(fdk_ctx *c, int dtype, const void *d_vol, int ang0, int nang,
