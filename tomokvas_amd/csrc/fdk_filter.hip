@@ -599,15 +599,19 @@ static int launch_filter_plan(const FilterLaunch &a, hipStream_t st)
     return FDK_OK;
 }
 
-// radix 16 over 256 threads; the longest rows (FP32 p = 16384, FP64 p = 8192)
-// one transform per workgroup of 1024 / 512 threads, 16 values per thread (the
-// transform alone fills 128 KiB of LDS); TKV_FILTER_R8: long FP32 rows
-// (p >= 2048) in radix-8 passes over 512 threads, 8 values per thread
+// radix 16 over 256 threads, at most 16 values per thread: FP32 p = 8192 one
+// transform per workgroup of 512 threads (over 256 threads, 32 values each
+// spilled registers: 33.7 against 25.9 ms for 3072-sample rows x 1.1 M), the
+// longest rows (FP32 p = 16384, FP64 p = 8192) one per workgroup of 1024 / 512
+// threads (the transform alone fills 128 KiB of LDS); TKV_FILTER_R8: long FP32
+// rows (p >= 2048) in radix-8 passes over 512 threads, 8 values per thread
 template <typename R, int P>
 static int launch_filter_p(const FilterLaunch &a, hipStream_t st)
 {
     if constexpr (sizeof(R) == 4 && P > 8192)
         return launch_filter_plan<R, Plan<P, 16, 1024>, TKV_FILTER_WAVES>(a, st);
+    else if constexpr (sizeof(R) == 4 && P == 8192)
+        return launch_filter_plan<R, Plan<P, 16, 512>, TKV_FILTER_WAVES>(a, st);
     else if constexpr (sizeof(R) == 8 && P > 4096)
         return launch_filter_plan<R, Plan<P, 16, 512>, 2>(a, st);
     else if constexpr (TKV_FILTER_R8 && sizeof(R) == 4 && P >= 2048)
