@@ -363,6 +363,32 @@ def test_low_precision_storage_interpolates_in_fp32(torch, geo):
             assert 1e-5 < ep <= 1e-3
 
 
+def test_weight_and_pack_on_rows_not_16_byte_aligned(torch):
+    # Nu = 96 rows take 16-byte accesses only when the array starts 16-byte
+    # aligned; a view one float into a buffer takes the scalar path and gives
+    # the same bits; a stored stack that is not 16-byte aligned is refused
+    g = scan(**WIDE)
+    st = stages(g)
+    rng = np.random.default_rng(9)
+    x = rng.standard_normal((g.Nv, g.Nang, g.Nu)).astype(np.float32)
+    aligned = torch.from_numpy(x).cuda()
+    buf = torch.empty(x.size + 1, dtype=torch.float32, device="cuda")
+    off = buf[1:].view(x.shape)
+    off.copy_(aligned)
+    assert off.data_ptr() % 16 == 4
+    st.weight(aligned)
+    st.weight(off)
+    assert torch.equal(aligned, off)
+    m = torch.from_numpy(np.abs(x).max(axis=(0, 2)).astype(np.float32).view(np.int32)).cuda()
+    s1, c1 = st.pack(aligned, _lib.F16, m)
+    s2, c2 = st.pack(off, _lib.F16, m)
+    assert torch.equal(s1, s2) and torch.equal(c1, c2)
+    nb = st.stack_bytes(_lib.F16)
+    bad = torch.empty(nb + 16, dtype=torch.uint8, device="cuda")[4:4 + nb]
+    with pytest.raises(RuntimeError, match="16-byte aligned"):
+        st.pack_into(aligned, _lib.F16, bad, c1.clone(), 0, m)
+
+
 @pytest.mark.parametrize("geo", ["odd", "wide"])
 def test_pack_writes_the_stored_stack_exactly(torch, geo):
     # [beta][v][pitch] rows: the samples (FP16: q * (1/scale) rounded once to
