@@ -132,6 +132,27 @@ def test_engine_degenerate_shapes(shape):
     assert [c[0] for c in sink.calls] == list(range(-(-g.Nz // g.B)))
 
 
+def test_storage_is_scale_invariant_and_zero_safe():
+    # FP16 stores value/scale per projection, BF16 and FP32 the values: inputs
+    # scaled by 1e-30 or 1e30 reconstruct to the scaled volume with the same
+    # relative error as unscaled ones (FP16's range alone would overflow or
+    # flush them); zero projections give an exactly zero volume (no 0/0 scale)
+    g = scan(**ODD)
+    sino = np.random.default_rng(11).standard_normal((g.Nv, g.Nang, g.Nu)).astype(np.float32)
+    ref64 = O.fdk(ogeom(g), sino.astype(np.float64), kind=0, f64=True)
+    for st, name in ((_lib.F32, "f32"), (_lib.F16, "f16"), (_lib.BF16, "bf16")):
+        base, _, _ = run_engine(g, sino, storage=st)
+        e0 = O.rel_rmse(base, ref64)
+        for k in (1e-30, 1e30):
+            v, _, _ = run_engine(g, (sino * np.float32(k)).astype(np.float32), storage=st)
+            assert np.all(np.isfinite(v)), (name, k)
+            e = O.rel_rmse(v, ref64 * k)
+            say(f"  {name} x{k:g}: rel RMSE {e:.2e} (unscaled {e0:.2e})")
+            assert e <= 2 * e0 + 1e-6, (name, k, e, e0)
+        z, _, _ = run_engine(g, np.zeros_like(sino), storage=st)
+        assert not np.any(z), name
+
+
 def test_engine_block_size_invariance(cfg1):
     g, sino = cfg1
     g = scan(64, 64, 30, 64, 64, 40, B=16)
