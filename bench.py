@@ -392,6 +392,91 @@ def config_leg(name, hipm, storages, steps, cpu_full=False, threads=None):
     return out
 
 
+def toeplitz_leg():
+    """The row filter as a Toeplitz GEMM on the matrix cores against the FFT, at
+    1024^3 from 1800 x 2048^2: the FFT filter's time over the whole stack (one
+    call, weight fused), relative error of the filtered rows of detector
+    rows [950, 1050) (weighted, the cfg3 data) against the engine's FP64 FFT
+    filter, with FP32, FP16 and BF16 operands (scaled into FP16 range, products
+    summed in FP32 by the GEMM; also with each row's mean taken out first), and
+    the time of the whole GEMM (1800 x 2048 rows times the 2048 x 2048 matrix)
+    through torch.matmul.  The matrix holds the filter's own response:
+    H[m][n] = k[n - m], read from the FP64 filter's output for unit impulses."""
+    import torch
+    from tomokvas_amd.stages import Stages
+    w = geometry.workloads()["cfg3"]
+    g = w.geom
+    fbp = FilteredBackProjection(0)
+    g.apply(fbp, None, None)
+    _lib.check(fbp.SetFilter(w.filter), "SetFilter")
+    st = Stages(fbp)
+    Nu = st.Nu
+    r0, r1 = 950, 1050
+    sino = st.phantom(w.ellipsoids())
+    st.poisson_log(sino, w.I0, w.seed)
+    x = sino[r0:r1].clone()
+    # the product's row filter (weight fused, FP32 FFT) over the whole stack
+    st.filter(sino, weight=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st.filter(sino, weight=True)
+    torch.cuda.synchronize()
+    fft_ms = 1e3 * (time.perf_counter() - t0)
+    del sino
+    torch.cuda.empty_cache()
+    g2 = geometry.workloads()["cfg3"].geom
+    g2.Nv, g2.cZ = r1 - r0, g.cZ - r0
+    fbp2 = FilteredBackProjection(0)
+    g2.apply(fbp2, None, None)
+    _lib.check(fbp2.SetFilter(w.filter), "SetFilter")
+    st2 = Stages(fbp2)
+    st2.weight(x)                                        # the engine's weight of those rows
+    x64 = x.double()
+    ref = st2.filter(x64.clone())                        # FP64 FFT filter
+    # impulse responses: row 0 from an impulse at u = 0 gives k[n], n >= 0;
+    # from u = Nu-1, k[n - Nu + 1]
+    imp = torch.zeros((r1 - r0, 1, Nu), dtype=torch.float64, device="cuda")
+    imp[0, 0, 0] = 1.0
+    imp[1, 0, Nu - 1] = 1.0
+    resp = st2.filter(imp)
+    lags = torch.cat([resp[1, 0, :Nu - 1], resp[0, 0, :]])   # k[-(Nu-1)] .. k[Nu-1]
+    n = torch.arange(Nu, device="cuda")
+    H = lags[(n[None, :] - n[:, None]) + Nu - 1]             # H[m][n] = k[n - m]
+    X2 = x64.reshape(-1, Nu)
+    out = {"workload": w.name, "rows": [r0, r1], "fft_filter_ms": fft_ms,
+           "fp64_gemm_rel": float(((X2 @ H).reshape(ref.shape) - ref).norm() / ref.norm())}
+    sx, sh = float(X2.abs().max()), float(H.abs().max())
+    mu = X2.mean(dim=1, keepdim=True)
+    d = X2 - mu
+    sd = float(d.abs().max())
+    for name, dt in (("fp32", torch.float32), ("fp16", torch.float16), ("bf16", torch.bfloat16)):
+        Hs = (H / sh).to(dt)
+        y = ((X2 / sx).to(dt) @ Hs).double().reshape(ref.shape) * (sx * sh)
+        out[f"{name}_rel"] = float((y - ref).norm() / ref.norm())
+        yd = ((d / sd).to(dt) @ Hs).double() * (sd * sh) + mu * H.sum(dim=0)[None, :]
+        out[f"{name}_rel_mean_removed"] = float((yd.reshape(ref.shape) - ref).norm() / ref.norm())
+    M = g.Nv * g.Nang
+    for name, dt in (("fp16", torch.float16), ("bf16", torch.bfloat16), ("fp32", torch.float32)):
+        chunk = 1 << 20
+        A = torch.randn(chunk, Nu, device="cuda").to(dt)
+        B = (H / sh).to(dt)
+        C = torch.empty(chunk, Nu, device="cuda", dtype=dt)
+        torch.matmul(A, B, out=C)
+        torch.cuda.synchronize()
+        nrep = (M + chunk - 1) // chunk
+        t0 = time.perf_counter()
+        for _ in range(nrep):
+            torch.matmul(A, B, out=C)
+        torch.cuda.synchronize()
+        t = (time.perf_counter() - t0) * M / (nrep * chunk)
+        out[f"{name}_gemm_ms"] = 1e3 * t
+        out[f"{name}_gemm_tflops"] = 2.0 * M * Nu * Nu / t / 1e12
+        del A, B, C
+    fbp2.close()
+    fbp.close()
+    return out
+
+
 def diff_leg(name, steps_chunked):
     """One training step of the differentiable form (tomokvas_amd.diff): loss =
     sum of squares over the central 256^3 crop of the volume (all of it below
@@ -487,7 +572,16 @@ def main():
     ap.add_argument("--diff-leg", metavar="WORKLOAD",
                     help="only run the differentiation-form training steps of WORKLOAD "
                          "(cfg4 or cfg5) and print their JSON")
+    ap.add_argument("--toeplitz-leg", action="store_true",
+                    help="only run the Toeplitz-GEMM row filter comparison and print its JSON")
+    ap.add_argument("--no-toeplitz", action="store_true",
+                    help="leave out the Toeplitz-GEMM row filter comparison")
     args = ap.parse_args()
+    if args.toeplitz_leg:
+        import torch
+        assert torch.cuda.is_available(), "the Toeplitz leg needs a HIP device"
+        print(json.dumps(toeplitz_leg()), flush=True)
+        return
     if args.diff_leg:
         import torch
         assert torch.cuda.is_available(), "the differentiation leg needs a HIP device"
@@ -694,6 +788,20 @@ def main():
             if r.returncode != 0:
                 raise RuntimeError(f"differentiation leg {name} failed ({r.returncode})")
             out["differentiation"][name] = json.loads(r.stdout.strip().splitlines()[-1])
+    if not args.no_toeplitz:
+        # the north star's MFMA alternative to the FFT row filter, measured (torch
+        # in a child process, as above)
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--toeplitz-leg"],
+                           capture_output=True, text=True, timeout=300)
+        sys.stderr.write(r.stderr)
+        if r.returncode != 0:
+            raise RuntimeError(f"Toeplitz leg failed ({r.returncode})")
+        tz = json.loads(r.stdout.strip().splitlines()[-1])
+        out["filter_alternatives"] = tz
+        log(f"Toeplitz GEMM row filter: FP16 {tz['fp16_gemm_ms']:.1f} ms "
+            f"({tz['fp16_gemm_tflops']:.0f} TFLOP/s), rel error {tz['fp16_rel']:.1e}; BF16 "
+            f"{tz['bf16_gemm_ms']:.1f} ms, {tz['bf16_rel']:.1e}; FP32 {tz['fp32_gemm_ms']:.1f} ms, "
+            f"{tz['fp32_rel']:.1e}; FFT filter {tz['fft_filter_ms']:.1f} ms")
     print(json.dumps(out), flush=True)
 
 
