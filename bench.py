@@ -238,16 +238,17 @@ def cpu_model() -> str:
     return "unknown"
 
 
-def cpu_leg(fbp, w, g, block_vols, hipm, threads):
-    """CPU baseline: the FP32 OpenMP mirror (timed) and the FP64 oracle on a
-    seeded 64^3 block of the workload (its detector rows only), and the cfg1
-    workload (128^3 from 180 x 128^2) reconstructed in full by the mirror."""
+def block_check(fbp, w, g, block, block_vols, hipm, threads, reps):
+    """CPU baseline and FP64 errors on a seeded voxel block of a workload, from
+    its detector rows only: the FP32 OpenMP mirror (weight + filter of the rows,
+    backprojection of the block; median of `reps`) and the FP64 oracle, and each
+    GPU volume's block (block_vols: storage name -> z-slab [n][Ny][Nx]) against
+    both.  Returns (timing dict, errors dict)."""
     import oracle as O
     os.environ["OMP_NUM_THREADS"] = str(threads)
-    x0, y0, z0, n = geometry.sub_cube(g, w.seed)
+    x0, y0, z0, n = block
     lo, hi = geometry.detector_rows_of_block(g, x0, y0, z0, n)
     rows = hi - lo
-    log(f"cpu leg: block x{x0} y{y0} z{z0} n{n}, detector rows [{lo},{hi})")
     raw = np.empty((rows, g.Nang, g.Nu), np.float32)
     src = fbp.device_sinogram_ptr() + lo * g.Nang * g.Nu * 4
     rc = hipm.hipMemcpy(raw.ctypes.data, C.c_void_p(src), raw.nbytes, 2)
@@ -259,10 +260,8 @@ def cpu_leg(fbp, w, g, block_vols, hipm, threads):
                              np.arange(x0, x0 + n), indexing="ij")
     pts = np.stack([xx.ravel(), yy.ravel(), zz.ravel()], axis=1).astype(np.int32)
     Hf = O.filter_spectrum(w.filter, g.Nu, g.tau)
-    # FP32 mirror: the timed CPU baseline (weight + filter of the rows + BP of
-    # the block), median of 3
     tf, tb = [], []
-    for _ in range(3):
+    for _ in range(reps):
         t0 = time.perf_counter()
         f32 = O.filter_rows(og, O.weight(og, raw), Hf)
         t1 = time.perf_counter()
@@ -271,7 +270,6 @@ def cpu_leg(fbp, w, g, block_vols, hipm, threads):
         tf.append(t1 - t0)
         tb.append(t2 - t1)
     t_f, t_b = float(np.median(tf)), float(np.median(tb))
-    # FP64 oracle on the same raw projections
     t2 = time.perf_counter()
     f64 = O.filter_rows(og, O.weight(og, raw.astype(np.float64), f64=True), Hf, f64=True)
     o64 = O.backproject_points(og, f64, pts, f64=True)
@@ -279,9 +277,34 @@ def cpu_leg(fbp, w, g, block_vols, hipm, threads):
     del f32, f64, raw
     upd = pts.shape[0] * g.Nang
     gups = upd / t_b / 1e9
-    # the whole workload, extrapolated per voxel-update and per filtered row
     bp_full = g.voxel_updates / (gups * 1e9)
     filt_full = t_f * (g.Nv / rows)
+    timing = {
+        "gups": gups, "block": [x0, y0, z0, n], "detector_rows": [lo, hi],
+        "voxel_updates": upd, "bp_seconds": t_b, "filter_seconds": t_f,
+        "fp64_oracle_seconds": t3 - t2,
+        "extrapolated_full_fdk_seconds": bp_full + filt_full,
+        "extrapolated_full_bp_seconds": bp_full,
+        "extrapolated_full_filter_seconds": filt_full,
+    }
+    errs = {"mirror_f32_vs_fp64": O.rel_rmse(m32, o64)}
+    for st, vol in block_vols.items():
+        sub = vol[:, y0:y0 + n, x0:x0 + n].reshape(-1)
+        errs[f"gpu_{st}_vs_fp64_oracle_block"] = O.rel_rmse(sub, o64)
+        if st == "f32":
+            errs["gpu_f32_vs_mirror_block"] = O.rel_rmse(sub, m32)
+    return timing, errs
+
+
+def cpu_leg(fbp, w, g, block_vols, hipm, threads):
+    """CPU baseline: the FP32 OpenMP mirror (timed) and the FP64 oracle on a
+    seeded 64^3 block of the workload (its detector rows only), and the cfg1
+    workload (128^3 from 180 x 128^2) reconstructed in full by the mirror."""
+    import oracle as O
+    block = geometry.sub_cube(g, w.seed)
+    log(f"cpu leg: block {block}")
+    bt, errs = block_check(fbp, w, g, block, block_vols, hipm, threads, 3)
+    x0, y0, z0, n = block
     # cfg1 in full: the mirror's FDK of the Shepp-Logan projections
     w1 = geometry.workloads()["cfg1"]
     g1 = w1.geom
@@ -291,26 +314,22 @@ def cpu_leg(fbp, w, g, block_vols, hipm, threads):
     O.fdk(o1, s1, kind=0)
     t5 = time.perf_counter()
     out = {
-        "value": gups, "unit": "GUPS", "cores": threads, "kind": "port",
+        "value": bt["gups"], "unit": "GUPS", "cores": threads, "kind": "port",
         "nproc": os.cpu_count(), "cpu_model": cpu_model(),
         "sample": (f"FP32 OpenMP mirror (gcc -O3 -march=native -fopenmp, {threads} threads), "
                    f"backprojection of a seeded {n}^3 voxel block (x{x0} y{y0} z{z0}) over all "
-                   f"{g.Nang} projections = {upd:.3e} voxel-updates, median of 3; weight+filter "
-                   f"of its {rows} detector rows x {g.Nang} projections {t_f:.2f} s"),
-        "bp_seconds": t_b, "filter_seconds": t_f, "fp64_oracle_seconds": t3 - t2,
-        "extrapolated_full_fdk_seconds": bp_full + filt_full,
-        "extrapolated_full_bp_seconds": bp_full,
-        "extrapolated_full_filter_seconds": filt_full,
+                   f"{g.Nang} projections = {bt['voxel_updates']:.3e} voxel-updates, median of "
+                   f"3; weight+filter of its {bt['detector_rows'][1] - bt['detector_rows'][0]} "
+                   f"detector rows x {g.Nang} projections {bt['filter_seconds']:.2f} s"),
+        "bp_seconds": bt["bp_seconds"], "filter_seconds": bt["filter_seconds"],
+        "fp64_oracle_seconds": bt["fp64_oracle_seconds"],
+        "extrapolated_full_fdk_seconds": bt["extrapolated_full_fdk_seconds"],
+        "extrapolated_full_bp_seconds": bt["extrapolated_full_bp_seconds"],
+        "extrapolated_full_filter_seconds": bt["extrapolated_full_filter_seconds"],
         "cfg1_full_fdk_seconds": t5 - t4,
         "cfg1_gups": g1.voxel_updates / (t5 - t4) / 1e9,
     }
-    errs = {"mirror_f32_vs_fp64": O.rel_rmse(m32, o64)}
-    for st, vol in block_vols.items():
-        sub = vol[:, y0:y0 + n, x0:x0 + n].reshape(-1)
-        errs[f"gpu_{st}_vs_fp64_oracle_block"] = O.rel_rmse(sub, o64)
-        if st == "f32":
-            errs["gpu_f32_vs_mirror_block"] = O.rel_rmse(sub, m32)
-    return out, errs, (x0, y0, z0, n)
+    return out, errs, block
 
 
 def fdk_steps(fbp, L, hipm, steps, warmup):
@@ -331,9 +350,11 @@ def fdk_steps(fbp, L, hipm, steps, warmup):
     return [float(np.mean(s[:, i])) for i in range(3)], 1e3 * float(np.mean(walls))
 
 
-def config_leg(name, hipm, storages, steps, cpu_full=False, threads=None):
+def config_leg(name, hipm, storages, steps, cpu_full=False, threads=None, cpu_block=False):
     """GUPS and FDK seconds of one workload per storage type, with each
     low-precision volume's relative RMSE against the FP32 volume (on the GPU).
+    cpu_block: also every storage type's error against the FP64 oracle and the
+    CPU FP32 mirror's time on a seeded voxel block (block_check, one run);
     cpu_full: also reconstruct the workload in full with the CPU FP32 mirror
     (one timed run) and compare the GPU FP32 volume with it."""
     L = _lib.load()
@@ -354,6 +375,8 @@ def config_leg(name, hipm, storages, steps, cpu_full=False, threads=None):
     ref = C.c_void_p()
     assert hipm.hipMalloc(C.byref(ref), nvol * 4) == 0
     out = {"workload": w.name, "voxel_updates": g.voxel_updates}
+    block = geometry.sub_cube(g, w.seed, n=min(64, g.Nx, g.Ny, g.Nz))
+    block_vols = {}
     for st in storages:
         _lib.check(fbp.SetStorage(st), "SetStorage")
         (f, pk, bp), wall = fdk_steps(fbp, L, hipm, steps, 1)
@@ -368,6 +391,18 @@ def config_leg(name, hipm, storages, steps, cpu_full=False, threads=None):
         out[NAMES[st]] = {"gups": g.voxel_updates / (bp * 1e-3) / 1e9, "fdk_s": (f + pk + bp) / 1e3,
                           "filter_ms": f, "pack_ms": pk, "bp_ms": bp,
                           "rel_rmse_vs_f32_volume": e}
+        if cpu_block:
+            z0, n = block[2], block[3]
+            slab = np.empty((n, g.Ny, g.Nx), np.float32)
+            assert hipm.hipMemcpy(slab.ctypes.data, C.c_void_p(dvol.value + z0 * g.Nx * g.Ny * 4),
+                                  slab.nbytes, 2) == 0
+            block_vols[NAMES[st]] = slab
+    if cpu_block:
+        bt, errs = block_check(fbp, w, g, block, block_vols, hipm, threads, 1)
+        out["cpu_block"] = bt
+        out["rel_rmse_block"] = errs
+        log(f"{name} block {block}: CPU mirror {bt['gups']:.2f} GUPS, errors " +
+            ", ".join(f"{k} {v:.1e}" for k, v in errs.items()))
     if cpu_full and _lib.F32 in storages:
         import oracle as O
         raw = np.empty((g.Nv, g.Nang, g.Nu), np.float32)
@@ -770,7 +805,8 @@ def main():
     if not args.no_configs:
         cpu2 = out["cpu_baseline"] is not None
         out["configs"] = {name: config_leg(name, hipm, order, min(args.steps, 3),
-                                           cpu_full=cpu2 and name == "cfg2", threads=args.threads)
+                                           cpu_full=cpu2 and name == "cfg2", threads=args.threads,
+                                           cpu_block=cpu2)
                           for name in ("cfg1", "cfg2", "cfg4", "cfg5") if name != args.workload}
         c2 = out["configs"].get("cfg2", {}).get("cpu_full")
         if c2:
