@@ -154,9 +154,12 @@ def detector_rows_of_block(g: ScanGeometry, x0: int, y0: int, z0: int, n: int):
     of the block x0..x0+n-1, y0.., z0.. reads, over all angles (two rows of
     margin each side); the cosine weight and the row filter act on rows
     independently, so these rows alone reproduce the block."""
-    ax = g.scale * (np.arange(x0, x0 + n) - g.cx)
-    ay = g.scale * (np.arange(y0, y0 + n) - g.cy)
-    az = g.scale * (np.array([z0, z0 + n - 1]) - (g.Nz - 1) / 2 - g.cz)
+    # voxel coordinates with the axis flips applied (kernel.cu:82-88, 398-400,
+    # 501-506, 821)
+    sx, sy, sz = (-1.0 if f else 1.0 for f in g.flips)
+    ax = sx * g.scale * (np.arange(x0, x0 + n) - g.cx)
+    ay = sy * g.scale * (np.arange(y0, y0 + n) - g.cy)
+    az = sz * g.scale * (np.array([z0, z0 + n - 1]) - (g.Nz - 1) / 2 - g.cz)
     b = np.arange(g.Nang) * g.dbeta + g.rot
     X, Y = np.meshgrid(ax, ay)
     S = X.ravel()[:, None] * np.cos(b)[None, :] + Y.ravel()[:, None] * np.sin(b)[None, :]
