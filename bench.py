@@ -253,9 +253,7 @@ def block_check(fbp, w, g, block, block_vols, hipm, threads, reps):
     src = fbp.device_sinogram_ptr() + lo * g.Nang * g.Nu * 4
     rc = hipm.hipMemcpy(raw.ctypes.data, C.c_void_p(src), raw.nbytes, 2)
     assert rc == 0, "hipMemcpy D2H failed"
-    gd = g.oracle_dict()
-    gd.update(Nv=rows, cZ=g.cZ - lo)
-    og = O.Geometry(**gd)
+    og = O.Geometry(**g.rows_oracle_dict(lo, hi))
     zz, yy, xx = np.meshgrid(np.arange(z0, z0 + n), np.arange(y0, y0 + n),
                              np.arange(x0, x0 + n), indexing="ij")
     pts = np.stack([xx.ravel(), yy.ravel(), zz.ravel()], axis=1).astype(np.int32)
@@ -265,14 +263,14 @@ def block_check(fbp, w, g, block, block_vols, hipm, threads, reps):
         t0 = time.perf_counter()
         f32 = O.filter_rows(og, O.weight(og, raw), Hf)
         t1 = time.perf_counter()
-        m32 = O.backproject_points(og, f32, pts)
+        m32 = O.backproject_points(og, f32, pts, row0=lo)
         t2 = time.perf_counter()
         tf.append(t1 - t0)
         tb.append(t2 - t1)
     t_f, t_b = float(np.median(tf)), float(np.median(tb))
     t2 = time.perf_counter()
     f64 = O.filter_rows(og, O.weight(og, raw.astype(np.float64), f64=True), Hf, f64=True)
-    o64 = O.backproject_points(og, f64, pts, f64=True)
+    o64 = O.backproject_points(og, f64, pts, f64=True, row0=lo)
     t3 = time.perf_counter()
     del f32, f64, raw
     upd = pts.shape[0] * g.Nang

@@ -117,8 +117,8 @@ def _load():
         "ofdk_filter_direct_f64": (None, [G, C.c_int, dp]),
         "ofdk_backproject_f32": (None, [G, fp, fp]),
         "ofdk_backproject_f64": (None, [G, dp, dp]),
-        "ofdk_backproject_points_f32": (None, [G, fp, C.c_int, ip, fp]),
-        "ofdk_backproject_points_f64": (None, [G, dp, C.c_int, ip, dp]),
+        "ofdk_backproject_points_f32": (None, [G, fp, C.c_int, C.c_int, ip, fp]),
+        "ofdk_backproject_points_f64": (None, [G, dp, C.c_int, C.c_int, ip, dp]),
         "ofdk_forward_project_f32": (None, [G, fp, fp]),
         "ofdk_forward_project_f64": (None, [G, dp, dp]),
         "ofdk_accumulate_f32": (None, [C.c_size_t, fp, fp]),
@@ -255,13 +255,16 @@ def backproject(geom, sino, f64=False):
     return vol
 
 
-def backproject_points(geom, sino, xyz, f64=False):
+def backproject_points(geom, sino, xyz, f64=False, row0=0):
+    """Backprojection at the voxels xyz (x, y, global z).  row0 > 0: sino holds
+    detector rows [row0, row0 + geom.Nv) of a taller detector, geom.cZ relative
+    to row0; row positions are those of the taller detector (bit-identical)."""
     dt = np.float64 if f64 else np.float32
     s = np.ascontiguousarray(sino, dt)
     pts = np.ascontiguousarray(xyz, np.int32).reshape(-1, 3)
     out = np.zeros(pts.shape[0], dt)
     f = _load().ofdk_backproject_points_f64 if f64 else _load().ofdk_backproject_points_f32
-    f(_g(geom), s, pts.shape[0], pts, out)
+    f(_g(geom), s, int(row0), pts.shape[0], pts, out)
     return out
 
 

@@ -77,7 +77,7 @@ void ofdk_filter_f32(const ofdk_geom *g, const double *Hf, float *sino);
 void ofdk_filter_complex_f32(const ofdk_geom *g, const double *Hre, const double *Him,
                              float *sino);
 void ofdk_backproject_f32(const ofdk_geom *g, const float *sino, float *vol);
-void ofdk_backproject_points_f32(const ofdk_geom *g, const float *sino, int npts,
+void ofdk_backproject_points_f32(const ofdk_geom *g, const float *sino, int row0, int npts,
                                  const int32_t *xyz, float *out);
 void ofdk_forward_project_f32(const ofdk_geom *g, const float *vol, float *sino);
 void ofdk_accumulate_f32(size_t n, const float *partial, float *slab);
@@ -90,7 +90,7 @@ void ofdk_filter_f64(const ofdk_geom *g, const double *Hf, double *sino);
 void ofdk_filter_complex_f64(const ofdk_geom *g, const double *Hre, const double *Him,
                              double *sino);
 void ofdk_backproject_f64(const ofdk_geom *g, const double *sino, double *vol);
-void ofdk_backproject_points_f64(const ofdk_geom *g, const double *sino, int npts,
+void ofdk_backproject_points_f64(const ofdk_geom *g, const double *sino, int row0, int npts,
                                  const int32_t *xyz, double *out);
 void ofdk_forward_project_f64(const ofdk_geom *g, const double *vol, double *sino);
 void ofdk_fdk_f64(const ofdk_geom *g, int kind, double *sino, double *vol);

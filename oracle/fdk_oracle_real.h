@@ -193,13 +193,20 @@ void SUF(ofdk_filter_complex)(const ofdk_geom *g, const double *Hre,
 /* ---- detector interpolation, fQ3D (kernel.cu:121-177) ------------------------- */
 /* Value of the filtered projection `beta` at (p + cX, f + cZ); zero outside the
  * closed detector rectangle [0,Nu-1] x [0,Nv-1]; ceil/floor formulation of the
- * reference (interpolation from the upper sample towards the lower one). */
+ * reference (interpolation from the upper sample towards the lower one).
+ * row0 > 0: sino holds rows [row0, row0 + Nv) of a taller detector and g->cZ is
+ * relative to row0; the row position is evaluated on the taller detector, as
+ * its full reconstruction does (f + cZ + row0), then shifted down by row0.  The
+ * shift is exact (the difference is a multiple of the position's ulp below
+ * the position itself), so the positions are bit-identical to the full
+ * detector's. */
 static inline REAL SUF(fq3d)(const ofdk_geom *g, const REAL *sino, int beta,
-                             REAL T, REAL U, REAL aZ)
+                             REAL T, REAL U, REAL aZ, int row0)
 {
     /* fQ3D(T*U, aZ*U, ...) then _p + cX, _f + cZ: contracted to fmas */
     REAL scP = SUF(ffma)(T, U, (REAL)g->cX);
-    REAL scF = SUF(ffma)(aZ, U, (REAL)g->cZ);
+    REAL scF = row0 ? SUF(ffma)(aZ, U, (REAL)g->cZ + (REAL)row0) - (REAL)row0
+                    : SUF(ffma)(aZ, U, (REAL)g->cZ);
     if (scP < 0 || scP > (REAL)(g->Nu - 1)) return 0;
     if (scF < 0 || scF > (REAL)(g->Nv - 1)) return 0;
     int af = (int)ceil(scF), bf = (int)floor(scF);
@@ -256,7 +263,7 @@ void SUF(ofdk_backproject)(const ofdk_geom *g, const REAL *sino, REAL *vol)
                         REAL S = SUF(ffma)(aX, c, aY * s);
                         REAL T = SUF(ffma)(aY, c, -(aX * s));
                         REAL U = D / (D - S);
-                        acc[x] = SUF(ffma)(U * U, SUF(fq3d)(g, sino, b, T, U, aZ), acc[x]);
+                        acc[x] = SUF(ffma)(U * U, SUF(fq3d)(g, sino, b, T, U, aZ, 0), acc[x]);
                     }
                 }
                 REAL *out = vol + ((size_t)z * g->Ny + y) * g->Nx;
@@ -271,8 +278,9 @@ void SUF(ofdk_backproject)(const ofdk_geom *g, const REAL *sino, REAL *vol)
 
 /* Same arithmetic as ofdk_backproject for an explicit list of voxels
  * (xyz[3*i+0..2] = x, y, global z); used for voxel subsamples of volumes too
- * large to reconstruct on the CPU. */
-void SUF(ofdk_backproject_points)(const ofdk_geom *g, const REAL *sino, int npts,
+ * large to reconstruct on the CPU.  row0: sino holds detector rows
+ * [row0, row0 + Nv) of a taller detector, g->cZ relative to row0 (see fq3d). */
+void SUF(ofdk_backproject_points)(const ofdk_geom *g, const REAL *sino, int row0, int npts,
                                   const int32_t *xyz, REAL *out)
 {
     const int p = ofdk_padded_length(g->Nu);
@@ -295,7 +303,7 @@ void SUF(ofdk_backproject_points)(const ofdk_geom *g, const REAL *sino, int npts
             REAL S = SUF(ffma)(aX, c, aY * s);
             REAL T = SUF(ffma)(aY, c, -(aX * s));
             REAL U = D / (D - S);
-            acc = SUF(ffma)(U * U, SUF(fq3d)(g, sino, b, T, U, aZ), acc);
+            acc = SUF(ffma)(U * U, SUF(fq3d)(g, sino, b, T, U, aZ, row0), acc);
         }
         out[i] = acc * fscale;
     }

@@ -330,7 +330,7 @@ static int launch_fp_t(const FPLaunch &L, hipStream_t st)
     const int zrun = sizeof(R) == 4 ? FP_ZR : ZR;
     long long nb = (long long)a.tiles_xy * ((L.Nz + zrun - 1) / zrun);
     if (nb == 0 || L.nang == 0) return FDK_OK;
-    if (nb > 0x7fffffffLL) return fail(FDK_ERR_UNSUPPORTED, "forward_project: volume too large");
+    if (!grid_fits(nb, 256)) return fail(FDK_ERR_UNSUPPORTED, "forward_project: volume too large");
     if constexpr (sizeof(R) == 4)
         hipLaunchKernelGGL(fp_lds_kernel, dim3((unsigned)nb), dim3(256), 0, st, a);
     else

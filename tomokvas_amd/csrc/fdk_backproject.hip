@@ -769,7 +769,7 @@ static int launch_bp_t(const BPLaunch &L, hipStream_t st)
     const long long nzrun = (L.Nz + ZR - 1) / ZR;
     const long long nb = (long long)a.tiles_xy * nzrun;
     if (nb == 0) return FDK_OK;
-    if (nb > 0x7fffffffLL) return fail(FDK_ERR_UNSUPPORTED, "backproject: volume too large");
+    if (!grid_fits(nb, 256)) return fail(FDK_ERR_UNSUPPORTED, "backproject: volume too large");
     a.nblocks = (int)nb;
     hipLaunchKernelGGL((bp_kernel<S, ZR, (ZR < TKV_BP_LB ? ZR : TKV_BP_LB)>), dim3((unsigned)nb),
                        dim3(256), 0, st, a);

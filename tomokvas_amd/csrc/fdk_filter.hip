@@ -103,7 +103,7 @@ int launch_weight(int real_bytes, void *sino, int Nu, int Nv, int nang, float cX
 {
     const long long rows = (long long)Nv * nang;
     if (rows == 0 || Nu == 0) return FDK_OK;
-    if ((rows + 3) / 4 > 0x7fffffffLL) return fail(FDK_ERR_UNSUPPORTED, "weight: too many rows");
+    if (!grid_fits((rows + 3) / 4, 256)) return fail(FDK_ERR_UNSUPPORTED, "weight: too many rows");
     const dim3 grid((unsigned)((rows + 3) / 4));
     const int vec4 = (Nu & 3) == 0 && ((uintptr_t)sino & 15) == 0;
     if (real_bytes == 8)
@@ -579,7 +579,7 @@ static int launch_filter_plan(const FilterLaunch &a, hipStream_t st)
     long long pairs = (rows + 1) / 2;
     long long blocks = (pairs + PL::NT - 1) / PL::NT;
     if (blocks == 0) return FDK_OK;
-    if (blocks > 0x7fffffffLL) return fail(FDK_ERR_UNSUPPORTED, "filter: too many rows");
+    if (!grid_fits(blocks, PL::BLOCK)) return fail(FDK_ERR_UNSUPPORTED, "filter: too many rows");
     if (2 * a.Nu > P) return fail(FDK_ERR_INVALID_ARGUMENT, "filter: rows longer than p/2");
     size_t lds = PL::NPASS > 1 || PL::RFIRST != PL::RLAST
                      ? sizeof(cpx<R>) * (size_t)PL::NT * Pad<R, PL>::SIZE
@@ -941,7 +941,7 @@ int launch_pack(int dtype_in, const void *filt, int Nu, int Nv, int nang, int pi
     if (nang == 0 || Nv == 0) return FDK_OK;
     if (pitch % 4 != 0) return fail(FDK_ERR_INVALID_ARGUMENT, "pack: pitch not a multiple of 4");
     const long long rows = (long long)nang * Nv;
-    if ((rows + 3) / 4 > 0x7fffffffLL) return fail(FDK_ERR_UNSUPPORTED, "pack: stack too large");
+    if (!grid_fits((rows + 3) / 4, 256)) return fail(FDK_ERR_UNSUPPORTED, "pack: stack too large");
     dim3 grid((unsigned)((rows + 3) / 4));
     const int vec4 = (Nu & 3) == 0 && ((uintptr_t)filt & 15) == 0;
     if (dtype_in == FDK_F64) {
@@ -1000,15 +1000,16 @@ int launch_accumulate(float *dst, const float *src, size_t n, hipStream_t st)
 // zero fill by a kernel (no memset node, so the sequence stays graph-capturable)
 __global__ void __launch_bounds__(256) zero_u32_kernel(unsigned *__restrict__ dst, size_t n)
 {
-    size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (i < n) dst[i] = 0u;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+        dst[i] = 0u;
 }
 
 int launch_zero_u32(unsigned *dst, size_t n, hipStream_t st)
 {
     if (n == 0) return FDK_OK;
-    hipLaunchKernelGGL(zero_u32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, dst,
-                       n);
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(zero_u32_kernel, dim3((unsigned)blocks), dim3(256), 0, st, dst, n);
     TKV_HIP(hipGetLastError());
     return FDK_OK;
 }

@@ -69,6 +69,15 @@ void build_spectrum(int kind, const Geometry &g, const std::vector<double> &user
 void fft_pow2(std::vector<double> &re, std::vector<double> &im);
 
 // ---------------------------------------------------------------- kernel launchers
+// A dispatch holds its size in work-items in a 32-bit field per dimension, so a
+// one-dimensional launch covers at most 2^32 - 1 threads; the launch call does
+// not reject a larger grid, which then falls short of its range.  Launchers
+// check their grids with this, or loop over a capped grid.
+inline bool grid_fits(long long blocks, int threads)
+{
+    return blocks >= 0 && (unsigned long long)blocks * (unsigned long long)threads <= 0xffffffffull;
+}
+
 struct FilterLaunch {
     const void *in;         // [rows][Nu] rows = Nv * nang, Real
     void *out;              // may alias in

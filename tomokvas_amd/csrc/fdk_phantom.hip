@@ -143,19 +143,21 @@ __device__ double poisson_draw(double lam, uint64_t seed, uint64_t idx)
 __global__ void __launch_bounds__(256)
     poisson_kernel(float *__restrict__ p, size_t n, double I0, uint64_t seed)
 {
-    size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    double lam = I0 * exp(-(double)p[i]);
-    double cnt = poisson_draw(lam, seed, (uint64_t)i);
-    if (cnt < 1.0) cnt = 1.0;
-    p[i] = (float)(-log(cnt / I0));
+    // grid-stride over a capped grid: every sample keeps its own stream index
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * 256) {
+        double lam = I0 * exp(-(double)p[i]);
+        double cnt = poisson_draw(lam, seed, (uint64_t)i);
+        if (cnt < 1.0) cnt = 1.0;
+        p[i] = (float)(-log(cnt / I0));
+    }
 }
 
 int launch_poisson_log(float *p, size_t n, double I0, uint64_t seed, hipStream_t st)
 {
     if (n == 0) return FDK_OK;
     size_t blocks = (n + 255) / 256;
-    if (blocks > 0x7fffffffull) return fail(FDK_ERR_UNSUPPORTED, "poisson: too many elements");
+    if (blocks > (1u << 20)) blocks = 1u << 20;   // 2^28 threads, within a dispatch's range
     hipLaunchKernelGGL(poisson_kernel, dim3((unsigned)blocks), dim3(256), 0, st, p, n, I0, seed);
     TKV_HIP(hipGetLastError());
     return FDK_OK;

@@ -83,6 +83,14 @@ class ScanGeometry:
                     scale=self.scale, flipX=self.flips[0], flipY=self.flips[1],
                     flipZ=self.flips[2], B=self.B)
 
+    def rows_oracle_dict(self, lo: int, hi: int) -> dict:
+        """oracle.Geometry arguments for detector rows [lo, hi) alone: the float
+        cZ the engine receives, less lo (an exact float difference), so the
+        weight of every row equals the full detector's bit for bit."""
+        d = self.oracle_dict()
+        d.update(Nv=hi - lo, cZ=float(np.float32(self.cZ)) - lo)
+        return d
+
 
 def flat_detector(Nu: int, Nv: int, Nang: int, pitch_mm: float, sod_mm: float, sdd_mm: float,
                   Nx: int, Ny: int, Nz: int, scale: float, **kw) -> ScanGeometry:
