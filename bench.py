@@ -428,13 +428,16 @@ def config_leg(name, hipm, storages, steps, cpu_full=False, threads=None, cpu_bl
 def toeplitz_leg():
     """The row filter as a Toeplitz GEMM on the matrix cores against the FFT, at
     1024^3 from 1800 x 2048^2: the FFT filter's time over the whole stack (one
-    call, weight fused), relative error of the filtered rows of detector
-    rows [950, 1050) (weighted, the cfg3 data) against the engine's FP64 FFT
-    filter, with FP32, FP16 and BF16 operands (scaled into FP16 range, products
-    summed in FP32 by the GEMM; also with each row's mean taken out first), and
-    the time of the whole GEMM (1800 x 2048 rows times the 2048 x 2048 matrix)
-    through torch.matmul.  The matrix holds the filter's own response:
-    H[m][n] = k[n - m], read from the FP64 filter's output for unit impulses."""
+    call, weight fused); the time of the whole GEMM (1800 x 2048 rows times the
+    2048 x 2048 matrix) through torch.matmul; and, on the detector rows of the
+    seeded central 64^3 block (the cfg3 data, weighted in FP32 by the engine),
+    the relative error of the filtered rows against the engine's FP64 filter
+    with FP32, FP16 and BF16 operands (scaled into FP16 range, products summed
+    in FP32 by the GEMM; also with each row's mean taken out first), and the
+    error each filter gives the block's FDK (packed in the matching storage
+    type, backprojected by the engine) against the engine's FP64 FDK of the
+    block.  The matrix holds the filter's own response: H[m][n] = k[n - m],
+    read from the FP64 filter's output for unit impulses."""
     import torch
     from tomokvas_amd.stages import Stages
     w = geometry.workloads()["cfg3"]
@@ -444,10 +447,11 @@ def toeplitz_leg():
     _lib.check(fbp.SetFilter(w.filter), "SetFilter")
     st = Stages(fbp)
     Nu = st.Nu
-    r0, r1 = 950, 1050
+    x0, y0, z0, n = geometry.sub_cube(g, w.seed)
+    r0, r1 = geometry.detector_rows_of_block(g, x0, y0, z0, n)
     sino = st.phantom(w.ellipsoids())
     st.poisson_log(sino, w.I0, w.seed)
-    x = sino[r0:r1].clone()
+    raw = sino[r0:r1].clone()
     # the product's row filter (weight fused, FP32 FFT) over the whole stack
     st.filter(sino, weight=True)
     torch.cuda.synchronize()
@@ -457,15 +461,40 @@ def toeplitz_leg():
     fft_ms = 1e3 * (time.perf_counter() - t0)
     del sino
     torch.cuda.empty_cache()
-    g2 = geometry.workloads()["cfg3"].geom
-    g2.Nv, g2.cZ = r1 - r0, g.cZ - r0
+    # an engine for detector rows [r0, r1) (the float cZ less r0, an exact
+    # difference) and a 64^3 volume whose voxels are the block's
+    gb = geometry.workloads()["cfg3"].geom
+    gb.Nv, gb.cZ = r1 - r0, float(np.float32(g.cZ)) - r0
+    gb.Nx = gb.Ny = gb.Nz = n
+    gb.cx, gb.cy = g.cx - x0, g.cy - y0
+    gb.cz = (g.Nz - 1) / 2 - (n - 1) / 2 - z0
     fbp2 = FilteredBackProjection(0)
-    g2.apply(fbp2, None, None)
+    gb.apply(fbp2, None, None)
     _lib.check(fbp2.SetFilter(w.filter), "SetFilter")
     st2 = Stages(fbp2)
-    st2.weight(x)                                        # the engine's weight of those rows
+    # the block's FDK in FP64 throughout (the engine's FP64 instantiation)
+    f64 = st2.filter(st2.weight(raw.double()))
+    s64, c64 = st2.pack(f64, _lib.F64)
+    v_ref = st2.backproject(s64, _lib.F64, c64)
+    del f64, s64, c64
+
+    def block_err(rows32, storage):
+        m = None
+        if storage != _lib.F32:
+            m = rows32.abs().amax(dim=(0, 2)).contiguous().view(torch.int32)
+        stk, sc = st2.pack(rows32.contiguous(), storage, m)
+        v = st2.backproject(stk, storage, sc).double()
+        return float((v - v_ref).norm() / v_ref.norm())
+
+    x = st2.weight(raw.clone())                          # the engine's FP32 weight
+    del raw
     x64 = x.double()
     ref = st2.filter(x64.clone())                        # FP64 FFT filter
+    fft32 = st2.filter(x.clone())                        # the product's FP32 FFT
+    fft_rel = float((fft32.double() - ref).norm() / ref.norm())
+    fdk = {"fft_f32": block_err(fft32, _lib.F32), "fft_f16": block_err(fft32, _lib.F16),
+           "fft_bf16": block_err(fft32, _lib.BF16)}
+    del fft32
     # impulse responses: row 0 from an impulse at u = 0 gives k[n], n >= 0;
     # from u = Nu-1, k[n - Nu + 1]
     imp = torch.zeros((r1 - r0, 1, Nu), dtype=torch.float64, device="cuda")
@@ -473,21 +502,33 @@ def toeplitz_leg():
     imp[1, 0, Nu - 1] = 1.0
     resp = st2.filter(imp)
     lags = torch.cat([resp[1, 0, :Nu - 1], resp[0, 0, :]])   # k[-(Nu-1)] .. k[Nu-1]
-    n = torch.arange(Nu, device="cuda")
-    H = lags[(n[None, :] - n[:, None]) + Nu - 1]             # H[m][n] = k[n - m]
+    ni = torch.arange(Nu, device="cuda")
+    H = lags[(ni[None, :] - ni[:, None]) + Nu - 1]           # H[m][n] = k[n - m]
     X2 = x64.reshape(-1, Nu)
-    out = {"workload": w.name, "rows": [r0, r1], "fft_filter_ms": fft_ms,
+    del x, x64
+    out = {"workload": w.name, "rows": [r0, r1], "block": [x0, y0, z0, n],
+           "fft_filter_ms": fft_ms, "fft_f32_rel": fft_rel,
            "fp64_gemm_rel": float(((X2 @ H).reshape(ref.shape) - ref).norm() / ref.norm())}
     sx, sh = float(X2.abs().max()), float(H.abs().max())
     mu = X2.mean(dim=1, keepdim=True)
     d = X2 - mu
     sd = float(d.abs().max())
-    for name, dt in (("fp32", torch.float32), ("fp16", torch.float16), ("bf16", torch.bfloat16)):
+    for name, dt, storage in (("fp32", torch.float32, _lib.F32),
+                              ("fp16", torch.float16, _lib.F16),
+                              ("bf16", torch.bfloat16, _lib.BF16)):
         Hs = (H / sh).to(dt)
         y = ((X2 / sx).to(dt) @ Hs).double().reshape(ref.shape) * (sx * sh)
         out[f"{name}_rel"] = float((y - ref).norm() / ref.norm())
-        yd = ((d / sd).to(dt) @ Hs).double() * (sd * sh) + mu * H.sum(dim=0)[None, :]
-        out[f"{name}_rel_mean_removed"] = float((yd.reshape(ref.shape) - ref).norm() / ref.norm())
+        fdk[f"toeplitz_{name}"] = block_err(y.float(), storage)
+        del y
+        yd = (((d / sd).to(dt) @ Hs).double() * (sd * sh) + mu * H.sum(dim=0)[None, :])
+        yd = yd.reshape(ref.shape)
+        out[f"{name}_rel_mean_removed"] = float((yd - ref).norm() / ref.norm())
+        fdk[f"toeplitz_{name}_mean_removed"] = block_err(yd.float(), storage)
+        del yd, Hs
+    out["fdk_block_rel_vs_fp64"] = fdk
+    del X2, d, ref, v_ref
+    torch.cuda.empty_cache()
     M = g.Nv * g.Nang
     for name, dt in (("fp16", torch.float16), ("bf16", torch.bfloat16), ("fp32", torch.float32)):
         chunk = 1 << 20
@@ -836,6 +877,8 @@ def main():
             f"({tz['fp16_gemm_tflops']:.0f} TFLOP/s), rel error {tz['fp16_rel']:.1e}; BF16 "
             f"{tz['bf16_gemm_ms']:.1f} ms, {tz['bf16_rel']:.1e}; FP32 {tz['fp32_gemm_ms']:.1f} ms, "
             f"{tz['fp32_rel']:.1e}; FFT filter {tz['fft_filter_ms']:.1f} ms")
+        log("  64^3 block FDK vs FP64: " + ", ".join(
+            f"{k} {v:.1e}" for k, v in tz["fdk_block_rel_vs_fp64"].items()))
     print(json.dumps(out), flush=True)
 
 
