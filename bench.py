@@ -294,14 +294,38 @@ def block_check(fbp, w, g, block, block_vols, hipm, threads, reps):
     return timing, errs
 
 
+def full_filter_stage(fbp, w, g, hipm, rows_per_slice=128):
+    """Seconds the FP32 mirror takes to weight and filter every detector row of
+    the resident projections, in slices of rows_per_slice detector rows (each
+    copied to the host untimed, then weighted and filtered as a whole)."""
+    import oracle as O
+    Hf = O.filter_spectrum(w.filter, g.Nu, g.tau)
+    buf = np.empty((rows_per_slice, g.Nang, g.Nu), np.float32)
+    total = 0.0
+    for lo in range(0, g.Nv, rows_per_slice):
+        hi = min(g.Nv, lo + rows_per_slice)
+        part = buf[:hi - lo]
+        src = fbp.device_sinogram_ptr() + lo * g.Nang * g.Nu * 4
+        assert hipm.hipMemcpy(part.ctypes.data, C.c_void_p(src), part.nbytes, 2) == 0
+        og = O.Geometry(**g.rows_oracle_dict(lo, hi))
+        t0 = time.perf_counter()
+        f = O.filter_rows(og, O.weight(og, part), Hf)
+        total += time.perf_counter() - t0
+        del f
+    return total
+
+
 def cpu_leg(fbp, w, g, block_vols, hipm, threads):
     """CPU baseline: the FP32 OpenMP mirror (timed) and the FP64 oracle on a
-    seeded 64^3 block of the workload (its detector rows only), and the cfg1
+    seeded 64^3 block of the workload (its detector rows only), the mirror's
+    weight + filter of every detector row (the full filter stage), and the cfg1
     workload (128^3 from 180 x 128^2) reconstructed in full by the mirror."""
     import oracle as O
     block = geometry.sub_cube(g, w.seed)
     log(f"cpu leg: block {block}")
     bt, errs = block_check(fbp, w, g, block, block_vols, hipm, threads, 3)
+    t_filter_full = full_filter_stage(fbp, w, g, hipm)
+    log(f"cpu leg: full filter stage {t_filter_full:.1f} s")
     x0, y0, z0, n = block
     # cfg1 in full: the mirror's FDK of the Shepp-Logan projections
     w1 = geometry.workloads()["cfg1"]
@@ -318,10 +342,13 @@ def cpu_leg(fbp, w, g, block_vols, hipm, threads):
                    f"backprojection of a seeded {n}^3 voxel block (x{x0} y{y0} z{z0}) over all "
                    f"{g.Nang} projections = {bt['voxel_updates']:.3e} voxel-updates, median of "
                    f"3; weight+filter of its {bt['detector_rows'][1] - bt['detector_rows'][0]} "
-                   f"detector rows x {g.Nang} projections {bt['filter_seconds']:.2f} s"),
+                   f"detector rows x {g.Nang} projections {bt['filter_seconds']:.2f} s; "
+                   f"weight+filter of all {g.Nv} detector rows {t_filter_full:.1f} s"),
         "bp_seconds": bt["bp_seconds"], "filter_seconds": bt["filter_seconds"],
         "fp64_oracle_seconds": bt["fp64_oracle_seconds"],
-        "extrapolated_full_fdk_seconds": bt["extrapolated_full_fdk_seconds"],
+        "full_filter_seconds": t_filter_full,
+        # the backprojection extrapolated from the block, the filter stage measured
+        "extrapolated_full_fdk_seconds": bt["extrapolated_full_bp_seconds"] + t_filter_full,
         "extrapolated_full_bp_seconds": bt["extrapolated_full_bp_seconds"],
         "extrapolated_full_filter_seconds": bt["extrapolated_full_filter_seconds"],
         "cfg1_full_fdk_seconds": t5 - t4,
@@ -352,7 +379,8 @@ def config_leg(name, hipm, storages, steps, cpu_full=False, threads=None, cpu_bl
     """GUPS and FDK seconds of one workload per storage type, with each
     low-precision volume's relative RMSE against the FP32 volume (on the GPU).
     cpu_block: also every storage type's error against the FP64 oracle and the
-    CPU FP32 mirror's time on a seeded voxel block (block_check, one run);
+    CPU FP32 mirror's time on a seeded voxel block (block_check, one run), and
+    for the 1024^3 workloads the mirror's full filter stage (one run);
     cpu_full: also reconstruct the workload in full with the CPU FP32 mirror
     (one timed run) and compare the GPU FP32 volume with it."""
     L = _lib.load()
@@ -397,6 +425,10 @@ def config_leg(name, hipm, storages, steps, cpu_full=False, threads=None, cpu_bl
             block_vols[NAMES[st]] = slab
     if cpu_block:
         bt, errs = block_check(fbp, w, g, block, block_vols, hipm, threads, 1)
+        if g.Nz >= 1024:
+            bt["full_filter_seconds"] = full_filter_stage(fbp, w, g, hipm)
+            bt["extrapolated_full_fdk_seconds"] = (bt["extrapolated_full_bp_seconds"]
+                                                   + bt["full_filter_seconds"])
         out["cpu_block"] = bt
         out["rel_rmse_block"] = errs
         log(f"{name} block {block}: CPU mirror {bt['gups']:.2f} GUPS, errors " +
