@@ -242,14 +242,22 @@ __global__ void __launch_bounds__(256) fp_lds_kernel(FPArgs<float> a)
             const bool inner = bw & 0x10000;
             char *lds = (char *)box;
             const int rowb = -bv0 * (4 * FP_P);
+            // The next column's record is read before this column's splats, and
+            // the column test stays a lane predicate (the record is uniform over
+            // the wave): read into a scalar register, it made every column wait
+            // for its record.  Zero voxels splat zeros rather than carry a
+            // per-column mask (64 masks spilled): 44 -> 37 ms for the adjoint of
+            // a 256^3 crop of 1024^3 over 1200 projections of 1536^2.
+            float4 rn = colrec[64 * wave];
 #pragma unroll
             for (int j = 0; j < 64; ++j) {
-                const float4 r = colrec[64 * wave + j];
-                const int cbj = __builtin_amdgcn_readfirstlane(__float_as_int(r.y));
-                if (cbj < 0) continue;   // column inactive at this angle
+                const float4 r = rn;
+                if (j + 1 < 64) rn = colrec[64 * wave + j + 1];
+                const int cbj = __float_as_int(r.y);
                 const float v = fmaf(azl, r.x, a.cZ);
                 const float g = val[j];
-                if (g == 0.f || (!inner && !(v >= 0.f && v <= a.vmax))) continue;
+                // cbj < 0: column inactive at this angle
+                if (cbj < 0 || (!inner && !(v >= 0.f && v <= a.vmax))) continue;
                 const float fv = __builtin_amdgcn_fractf(v);
                 int *p0 = (int *)(lds + (int)v * (4 * FP_P) + rowb + cbj);
                 const float x1 = g * fv, x0 = g - x1;
