@@ -179,11 +179,11 @@ hipStream_t pick(fdk_ctx *, void *stream)
 }
 
 // workgroups of the tap-gradient kernel that give every one a transform of
-// its own: row pairs / transforms per workgroup (256 threads, min(p/4, 256) per
-// transform)
+// its own: row pairs / transforms per workgroup (256 threads; the radix-16
+// plan gives a transform min(p/16, 256) threads, one below p = 16)
 long long tap_blocks(int p, long long rows)
 {
-    const int tpt = std::min(p / 4, 256), nt = 256 / tpt;
+    const int tpt = p < 16 ? 1 : std::min(p / 16, 256), nt = 256 / tpt;
     return ((rows + 1) / 2 + nt - 1) / nt;
 }
 

@@ -117,98 +117,6 @@ int launch_weight(int real_bytes, void *sino, int Nu, int Nv, int nang, float cX
 }
 
 // ---------------------------------------------------------------- Stockham FFT
-template <int P>
-struct FFTShape {
-    static constexpr int TPT = (P / 4 < 256) ? P / 4 : 256;  // threads per transform
-    static constexpr int NT = 256 / TPT;                     // transforms per workgroup
-    static constexpr int BPT = (P / 4) / TPT;                // radix-4 butterflies per thread
-    static constexpr int LOG2 = __builtin_ctz(P);
-    static constexpr bool ODD = (LOG2 & 1) != 0;
-};
-
-// One Stockham pass of radix 4 over the transform in `s` (length P).  `pc` is
-// the length of the sub-transforms already done; inputs of butterfly i at
-// i + m*P/4, outputs at ((i - k) << 2) + k + m*pc with k = i mod pc.
-template <typename R, int P, bool INV>
-__device__ __forceinline__ void radix4_pass(cpx<R> *s, const cpx<R> *__restrict__ tw, int t,
-                                            int pc)
-{
-    using S = FFTShape<P>;
-    cpx<R> u[S::BPT][4];
-#pragma unroll
-    for (int b = 0; b < S::BPT; ++b) {
-        int i = t + b * S::TPT;
-#pragma unroll
-        for (int m = 0; m < 4; ++m) u[b][m] = s[i + m * (P / 4)];
-    }
-    __syncthreads();
-    const int stride = P / (4 * pc);
-#pragma unroll
-    for (int b = 0; b < S::BPT; ++b) {
-        int i = t + b * S::TPT;
-        int k = i & (pc - 1);
-        if (pc > 1) {
-#pragma unroll
-            for (int m = 1; m < 4; ++m) {
-                cpx<R> w = tw[m * k * stride];
-                if (INV) w.y = -w.y;
-                u[b][m] = cmul(u[b][m], w);
-            }
-        }
-        cpx<R> v0 = {u[b][0].x + u[b][2].x, u[b][0].y + u[b][2].y};
-        cpx<R> v1 = {u[b][0].x - u[b][2].x, u[b][0].y - u[b][2].y};
-        cpx<R> v2 = {u[b][1].x + u[b][3].x, u[b][1].y + u[b][3].y};
-        cpx<R> d = {u[b][1].x - u[b][3].x, u[b][1].y - u[b][3].y};
-        // forward: d * (-i) = (d.y, -d.x); inverse: d * (+i) = (-d.y, d.x)
-        cpx<R> v3 = INV ? cpx<R>{-d.y, d.x} : cpx<R>{d.y, -d.x};
-        int j = ((i - k) << 2) + k;
-        s[j] = {v0.x + v2.x, v0.y + v2.y};
-        s[j + pc] = {v1.x + v3.x, v1.y + v3.y};
-        s[j + 2 * pc] = {v0.x - v2.x, v0.y - v2.y};
-        s[j + 3 * pc] = {v1.x - v3.x, v1.y - v3.y};
-    }
-    __syncthreads();
-}
-
-// First pass of radix 2 (pc = 1, trivial twiddles): outputs 2i, 2i+1.
-template <typename R, int P>
-__device__ __forceinline__ void radix2_first(cpx<R> *s, int t)
-{
-    using S = FFTShape<P>;
-    constexpr int NB = (P / 2) / S::TPT;
-    cpx<R> u[NB][2];
-#pragma unroll
-    for (int b = 0; b < NB; ++b) {
-        int i = t + b * S::TPT;
-        u[b][0] = s[i];
-        u[b][1] = s[i + P / 2];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int b = 0; b < NB; ++b) {
-        int i = t + b * S::TPT;
-        s[2 * i] = {u[b][0].x + u[b][1].x, u[b][0].y + u[b][1].y};
-        s[2 * i + 1] = {u[b][0].x - u[b][1].x, u[b][0].y - u[b][1].y};
-    }
-    __syncthreads();
-}
-
-template <typename R, int P, bool INV>
-__device__ __forceinline__ void fft_lds(cpx<R> *s, const cpx<R> *__restrict__ tw, int t)
-{
-    using S = FFTShape<P>;
-    int pc = 1;
-    if (S::ODD) {
-        radix2_first<R, P>(s, t);
-        pc = 2;
-    }
-#pragma unroll
-    for (int pass = 0; pass < S::LOG2 / 2; ++pass) {
-        radix4_pass<R, P, INV>(s, tw, t, pc);
-        pc *= 4;
-    }
-}
-
 template <typename R>
 __device__ __forceinline__ R group_max(R m, int width)
 {
@@ -661,64 +569,90 @@ int launch_filter(int real_bytes, const FilterLaunch &a, hipStream_t st)
 // registers and writes one partial spectrum; tapgrad_reduce_kernel sums the
 // partials in double, and the host symmetrises and applies the final length-p
 // transform (an FFT in double).
-template <typename R, int P>
-__global__ void __launch_bounds__(256)
+// Per transform of the workgroup (the row filter's radix-16 Stockham plan PL),
+// over row pairs pr = blockIdx.x*NT + f, stepping by gridDim.x*NT: the forward
+// transforms of the q pair and of the G pair leave their bins in the same
+// register slots, so P[k] accumulates slot by slot in registers; at the end the
+// NT transforms' sums are added through LDS in natural bin order.
+template <typename R, typename PL>
+__device__ __forceinline__ void tapgrad_load_pair(cpx<R> *reg, const R *__restrict__ x, int Nu,
+                                                  long long rA, bool hasA, bool hasB, int t)
+{
+    constexpr int P = PL::P;
+    const R *xa = x + (hasA ? rA : 0) * Nu;
+    const R *xb = xa + Nu;
+#pragma unroll
+    for (int b = 0; b < PL::E / PL::RFIRST; ++b) {
+        const int i = t + b * PL::TPT;
+#pragma unroll
+        for (int m = 0; m < PL::RFIRST; ++m) {
+            const int n = i + m * (P / PL::RFIRST);
+            R va = 0, vb = 0;
+            // Nu <= P/2: inputs m >= RFIRST/2 are zero at compile time
+            if (m < (PL::RFIRST + 1) / 2 && n < Nu) {
+                if (hasA) va = xa[n];
+                if (hasB) vb = xb[n];
+            }
+            reg[b * PL::RFIRST + m] = {va, vb};
+        }
+    }
+}
+
+template <typename R, typename PL>
+__global__ void __launch_bounds__(PL::BLOCK)
     tapgrad_kernel(const R *__restrict__ q, const R *__restrict__ G, int Nu, long long rows,
                    const cpx<R> *__restrict__ tw, cpx<R> *__restrict__ partial)
 {
-    using S = FFTShape<P>;
+    constexpr int P = PL::P;
+    constexpr int E = PL::E;
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    cpx<R> *smem = reinterpret_cast<cpx<R> *>(smem_raw);
     const int tid = threadIdx.x;
-    const int f = tid / S::TPT;
-    const int t = tid - f * S::TPT;
-    cpx<R> *sq = smem + f * P;
-    cpx<R> *sg = smem + (S::NT + f) * P;
-    constexpr int E = P / S::TPT;
+    const int f = tid / PL::TPT;
+    const int t = tid - f * PL::TPT;
+    cpx<R> *s = reinterpret_cast<cpx<R> *>(smem_raw) + f * Pad<R, PL>::SIZE;
     cpx<R> acc[E];
 #pragma unroll
-    for (int j = 0; j < E; ++j) acc[j] = {0, 0};
+    for (int e = 0; e < E; ++e) acc[e] = {0, 0};
     const long long pairs = (rows + 1) / 2;
-    for (long long pr0 = (long long)blockIdx.x * S::NT; pr0 < pairs;
-         pr0 += (long long)gridDim.x * S::NT) {
+    // the bound is the same for every thread of the workgroup (the passes'
+    // barriers are workgroup-wide); a transform past the last pair adds zeros
+    for (long long pr0 = (long long)blockIdx.x * PL::NT; pr0 < pairs;
+         pr0 += (long long)gridDim.x * PL::NT) {
         const long long pr = pr0 + f;
-        const long long rA = 2 * pr, rB = rA + 1;
-        const bool hasA = pr < pairs && rA < rows, hasB = pr < pairs && rB < rows;
+        const long long rA = 2 * pr;
+        const bool hasA = pr < pairs && rA < rows, hasB = pr < pairs && rA + 1 < rows;
+        cpx<R> zq[E], reg[E];
+        tapgrad_load_pair<R, PL>(reg, q, Nu, rA, hasA, hasB, t);
+        Passes<R, PL, false, 0>::run(reg, s, tw, t, 1);
 #pragma unroll
-        for (int j = 0; j < E; ++j) {
-            int n = t + j * S::TPT;
-            R qa = 0, qb = 0, ga = 0, gb = 0;
-            if (n < Nu) {
-                if (hasA) { qa = q[rA * Nu + n]; ga = G[rA * Nu + n]; }
-                if (hasB) { qb = q[rB * Nu + n]; gb = G[rB * Nu + n]; }
-            }
-            sq[n] = {qa, qb};
-            sg[n] = {ga, gb};
-        }
-        __syncthreads();
-        fft_lds<R, P, false>(sq, tw, t);
-        fft_lds<R, P, false>(sg, tw, t);
+        for (int e = 0; e < E; ++e) zq[e] = reg[e];
+        tapgrad_load_pair<R, PL>(reg, G, Nu, rA, hasA, hasB, t);
+        Passes<R, PL, false, 0>::run(reg, s, tw, t, 1);
 #pragma unroll
-        for (int j = 0; j < E; ++j) {
-            int n = t + j * S::TPT;
-            cpx<R> a = sq[n], g = sg[n];
-            acc[j].x += a.x * g.x + a.y * g.y;   // a * conj(g)
-            acc[j].y += a.y * g.x - a.x * g.y;
+        for (int e = 0; e < E; ++e) {
+            acc[e].x += zq[e].x * reg[e].x + zq[e].y * reg[e].y;   // Zq * conj(ZG)
+            acc[e].y += zq[e].y * reg[e].x - zq[e].x * reg[e].y;
         }
-        __syncthreads();
     }
-    // reduce the NT transforms of the workgroup into transform 0's layout
-    cpx<R> *red = smem;
+    // bins in natural order per transform, then the NT transforms' sum
+    using DL = DFT<R, PL::RLAST, false>;
+    cpx<R> *red = reinterpret_cast<cpx<R> *>(smem_raw);
+    __syncthreads();   // every transform is done with its LDS
 #pragma unroll
-    for (int j = 0; j < E; ++j) red[f * P + t + j * S::TPT] = acc[j];
+    for (int b = 0; b < E / PL::RLAST; ++b) {
+        const int i = t + b * PL::TPT;
+#pragma unroll
+        for (int m = 0; m < PL::RLAST; ++m)
+            red[f * P + i + m * (P / PL::RLAST)] = acc[b * PL::RLAST + DL::pos(m)];
+    }
     __syncthreads();
-    for (int n = tid; n < P; n += 256) {
-        cpx<R> s = {0, 0};
-        for (int ff = 0; ff < S::NT; ++ff) {
-            s.x += red[ff * P + n].x;
-            s.y += red[ff * P + n].y;
+    for (int n = tid; n < P; n += PL::BLOCK) {
+        cpx<R> sum = {0, 0};
+        for (int ff = 0; ff < PL::NT; ++ff) {
+            sum.x += red[ff * P + n].x;
+            sum.y += red[ff * P + n].y;
         }
-        partial[(size_t)blockIdx.x * P + n] = s;
+        partial[(size_t)blockIdx.x * P + n] = sum;
     }
 }
 
@@ -758,13 +692,16 @@ __global__ void __launch_bounds__(1024)
 template <typename R, int P>
 static int launch_tapgrad_p(const TapGradLaunch &a, hipStream_t st)
 {
-    using S = FFTShape<P>;
-    size_t lds = sizeof(cpx<R>) * (size_t)S::NT * P * 2;
-    auto kern = tapgrad_kernel<R, P>;
-    if (lds > 65536) TKV_HIP(hipFuncSetAttribute((const void *)kern,
-                                                  hipFuncAttributeMaxDynamicSharedMemorySize,
-                                                  (int)lds));
-    hipLaunchKernelGGL(kern, dim3((unsigned)a.nblocks), dim3(256), lds, st, (const R *)a.q,
+    using PL = Plan<P, 16, 256>;
+    // the passes' transforms, and the final sum in natural order
+    const size_t lds = sizeof(cpx<R>) * (size_t)PL::NT *
+                       (Pad<R, PL>::SIZE > P ? (size_t)Pad<R, PL>::SIZE : (size_t)P);
+    auto kern = tapgrad_kernel<R, PL>;
+    if (lds > 163840) return fail(FDK_ERR_UNSUPPORTED, "tap gradient: transform does not fit LDS");
+    if (lds > 65536)
+        TKV_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)lds));
+    hipLaunchKernelGGL(kern, dim3((unsigned)a.nblocks), dim3(PL::BLOCK), lds, st, (const R *)a.q,
                        (const R *)a.G, a.Nu, (long long)a.rows, (const cpx<R> *)a.tw,
                        (cpx<R> *)a.partial);
     TKV_HIP(hipGetLastError());
