@@ -124,7 +124,9 @@ int fdk_initialize(fdk_ctx *ctx);
  * backprojection, then for each slab n = 0..ceil(Nz/B)-1 the callback is asked
  * for its host buffer and the slab's slices are copied into it.  The reference
  * visits ceil((Nz-1)/B) slabs and so skips the last slice when (Nz-1) % B == 0;
- * this engine always covers every slice. */
+ * this engine always covers every slice.  Ordered after the work queued on the
+ * NULL stream before the call (e.g. filling fdk_device_sinogram); work on
+ * other streams must be complete.  Returns when the volume is finished. */
 int fdk_calculate_reconstruction(fdk_ctx *ctx, fdk_get_block_fn get_block, void *user);
 /* extension: fdk_initialize + fdk_calculate_reconstruction in one call with
  * the host stack [v][beta][u] borrowed for its duration (set the sinogram
@@ -133,14 +135,16 @@ int fdk_calculate_reconstruction(fdk_ctx *ctx, fdk_get_block_fn get_block, void 
  * is filtered and backprojected, so the host-link transfer hides behind the
  * computation (page-locked host memory transfers fastest).  Stage times:
  * [0], [1] zero, [2] first upload to last backprojection, [3] slab transfers
- * left after it.  fdk_device_sinogram's contents are unspecified afterwards. */
+ * left after it.  fdk_device_sinogram's contents are unspecified afterwards.
+ * Ordered after the NULL stream's earlier work, as fdk_calculate_reconstruction. */
 int fdk_reconstruct_host(fdk_ctx *ctx, const float *host_sino, fdk_get_block_fn get_block,
                          void *user);
 /* Device copy of the last reconstructed volume [Nz][Ny][Nx] (owned by ctx). */
 const float *fdk_device_volume(fdk_ctx *ctx);
 /* Device projection stack [v][beta][u] of an initialised context.  With a NULL
  * sinogram pointer fdk_initialize allocates it without an upload, so callers
- * whose projections are already in HBM fill it here. */
+ * whose projections are already in HBM fill it here (on the NULL stream, or
+ * with their own streams synchronised before fdk_calculate_reconstruction). */
 float *fdk_device_sinogram(fdk_ctx *ctx);
 /* Stage times of the last fdk_calculate_reconstruction in ms (device events):
  * [0] cosine weight + ramp filter, [1] pack to the storage type,

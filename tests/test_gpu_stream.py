@@ -23,10 +23,15 @@ pytestmark = pytest.mark.gpu
 _hip = None
 
 
-def d2h(ptr, shape):
+def hip_runtime():
     global _hip
     if _hip is None:
         _hip = C.CDLL("libamdhip64.so")
+    return _hip
+
+
+def d2h(ptr, shape):
+    hip_runtime()
     out = np.empty(shape, np.float32)
     assert _hip.hipMemcpy(C.c_void_p(out.ctypes.data), C.c_void_p(ptr), C.c_size_t(out.nbytes),
                           2) == 0
@@ -242,3 +247,36 @@ def test_setters_after_initialize_require_it_again():
     fbp.close()
     want = one_launch(big, s, _lib.F32)
     np.testing.assert_array_equal(sink2.volume, want)
+
+
+def test_calculate_waits_for_the_null_stream():
+    """The context's streams are non-blocking: work the caller queued on the
+    NULL stream before fdk_calculate_reconstruction -- here the noise kernel
+    over the 9.4e8 samples of the 512^3 workload's projections, left running
+    asynchronously -- is finished before the filter reads the stack, so the
+    volume equals the one computed with the device synchronised in between."""
+    from tomokvas_amd import geometry
+    from tomokvas_amd.engine import FilteredBackProjection
+    w = geometry.workloads()["cfg2"]
+    g = w.geom
+    L = _lib.load()
+    say("512^3 from 900 x 1024^2: noise left running on the NULL stream, then the FDK")
+    vols = []
+    for sync in (True, False):
+        fbp = FilteredBackProjection(0)
+        g.apply(fbp, None, None)
+        _lib.check(fbp.SetFilter(w.filter), "SetFilter")
+        _lib.check(fbp.Initialize(), "Initialize")
+        ells = np.ascontiguousarray(w.ellipsoids())
+        _lib.check(L.fdk_phantom_project(fbp.handle, ells.ctypes.data, ells.shape[0], 0, g.Nang,
+                                         fbp.device_sinogram_ptr(), None), "phantom")
+        _lib.check(L.fdk_poisson_log(fbp.device_sinogram_ptr(), g.Nu * g.Nv * g.Nang, w.I0,
+                                     w.seed, None), "poisson")
+        if sync:
+            assert hip_runtime().hipDeviceSynchronize() == 0
+        _lib.check(L.fdk_calculate_reconstruction(fbp.handle, _lib.GET_BLOCK_FN(), None),
+                   "calculate")
+        vols.append(d2h(fbp.device_volume_ptr(), (g.Nz, g.Ny, g.Nx)))
+        fbp.close()
+    assert np.abs(vols[0]).max() > 0
+    assert np.array_equal(vols[0], vols[1])

@@ -78,6 +78,7 @@ struct fdk_ctx {
     hipStream_t copy_stream = nullptr;
     hipStream_t prep_stream = nullptr;   // fdk_reconstruct_host: filter and pack of a chunk
     hipEvent_t ev_bp[2] = {}, ev_cp[2] = {}, ev_up[2] = {}, ev_f[2] = {};
+    hipEvent_t ev_caller = nullptr;   // the caller's NULL-stream work (after_caller_work)
     int ring_slices = 0;     // streamed volume: slices held in HBM (circular)
 
     // tables (prepared)
@@ -580,12 +581,27 @@ int create_streams(fdk_ctx *c)
     TKV_HIP(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
     TKV_HIP(hipStreamCreateWithFlags(&c->prep_stream, hipStreamNonBlocking));
     for (auto &e : c->ev) TKV_HIP(hipEventCreate(&e));
+    TKV_HIP(hipEventCreateWithFlags(&c->ev_caller, hipEventDisableTiming));
     for (int i = 0; i < 2; ++i) {
         TKV_HIP(hipEventCreateWithFlags(&c->ev_bp[i], hipEventDisableTiming));
         TKV_HIP(hipEventCreateWithFlags(&c->ev_cp[i], hipEventDisableTiming));
         TKV_HIP(hipEventCreateWithFlags(&c->ev_up[i], hipEventDisableTiming));
         TKV_HIP(hipEventCreateWithFlags(&c->ev_f[i], hipEventDisableTiming));
     }
+    return FDK_OK;
+}
+
+// The context's streams are non-blocking, so nothing orders them after work
+// the caller queued on the NULL stream before a call -- filling the device
+// projection stack through stage-level calls or torch, say.  Entry points that
+// run on the context's streams first make each of them wait for an event
+// recorded on the NULL stream.
+int after_caller_work(fdk_ctx *c)
+{
+    TKV_HIP(hipEventRecord(c->ev_caller, nullptr));
+    TKV_HIP(hipStreamWaitEvent(c->stream, c->ev_caller, 0));
+    TKV_HIP(hipStreamWaitEvent(c->copy_stream, c->ev_caller, 0));
+    TKV_HIP(hipStreamWaitEvent(c->prep_stream, c->ev_caller, 0));
     return FDK_OK;
 }
 
@@ -602,6 +618,7 @@ void destroy_streams(fdk_ctx *c)
         s = nullptr;
     };
     for (auto &e : c->ev) drop_event(e);
+    drop_event(c->ev_caller);
     for (int i = 0; i < 2; ++i) {
         drop_event(c->ev_bp[i]);
         drop_event(c->ev_cp[i]);
@@ -909,6 +926,7 @@ int fdk_calculate_reconstruction(fdk_ctx *c, fdk_get_block_fn get_block, void *u
     if (!c->initialized) return fail(FDK_ERR_NOT_INITIALIZED, "call fdk_initialize first");
     int rc = set_device(c);
     if (rc) return rc;
+    if ((rc = after_caller_work(c))) return rc;
     const tkv::Geometry &g = c->g;
     hipStream_t st = c->stream;
     if ((rc = tkv::launch_zero_u32(c->d_maxabs, g.Nang, st))) return rc;
@@ -982,6 +1000,7 @@ int fdk_reconstruct_host(fdk_ctx *c, const float *host_sino, fdk_get_block_fn ge
     int rc = fdk_initialize(c);
     if (rc) return rc;
     if ((rc = set_device(c))) return rc;
+    if ((rc = after_caller_work(c))) return rc;
     const tkv::Geometry &g = c->g;
     hipStream_t st = c->stream, cs = c->copy_stream;
     const size_t per_ang = (size_t)g.Nu * g.Nv;
