@@ -386,14 +386,26 @@ __global__ void __launch_bounds__(PL::BLOCK) __attribute__((amdgpu_waves_per_eu(
 #pragma unroll
     for (int b = 0; b < E / PL::RFIRST; ++b) {
         const int i = t + b * PL::TPT;
+        // all of the butterfly's loads first (column clamped into the row; a
+        // missing row is row 0), masked after: they are in flight together
+        // while the weights are formed, not one pair per round trip
+        constexpr int MH = (PL::RFIRST + 1) / 2;
+        R la[MH], lc[MH];
+#pragma unroll
+        for (int m = 0; m < MH; ++m) {
+            const int n = i + m * (P / PL::RFIRST);
+            const int nc = n < Nu ? n : Nu - 1;
+            la[m] = inA[nc];
+            lc[m] = inB[nc];
+        }
 #pragma unroll
         for (int m = 0; m < PL::RFIRST; ++m) {
             const int n = i + m * (P / PL::RFIRST);
             R a = 0, c = 0;
-            if (m < (PL::RFIRST + 1) / 2 && n < Nu) {
-                if (hasA) a = inA[n];
-                if (hasB) c = inB[n];
-                if (weight) {
+            if (m < MH) {
+                a = hasA && n < Nu ? la[m] : (R)0;
+                c = hasB && n < Nu ? lc[m] : (R)0;
+                if (weight) {   // finite for every n < P/2; zeros stay zero
                     const R wa = cos_weight<R>(n, vA, cX, cZ, D);
                     a *= wa;
                     if (same_v)
@@ -580,18 +592,29 @@ __device__ __forceinline__ void tapgrad_load_pair(cpx<R> *reg, const R *__restri
 {
     constexpr int P = PL::P;
     const R *xa = x + (hasA ? rA : 0) * Nu;
-    const R *xb = xa + Nu;
+    const R *xb = x + (hasB ? rA + 1 : 0) * Nu;
 #pragma unroll
     for (int b = 0; b < PL::E / PL::RFIRST; ++b) {
         const int i = t + b * PL::TPT;
+        // Nu <= P/2: inputs m >= RFIRST/2 are zero at compile time.  The
+        // loads are unconditional (column clamped into the row, a missing row
+        // read as row 0) and masked after, so they are in flight together
+        constexpr int MH = (PL::RFIRST + 1) / 2;
+        R la[MH], lb[MH];
+#pragma unroll
+        for (int m = 0; m < MH; ++m) {
+            const int n = i + m * (P / PL::RFIRST);
+            const int nc = n < Nu ? n : Nu - 1;
+            la[m] = xa[nc];
+            lb[m] = xb[nc];
+        }
 #pragma unroll
         for (int m = 0; m < PL::RFIRST; ++m) {
             const int n = i + m * (P / PL::RFIRST);
             R va = 0, vb = 0;
-            // Nu <= P/2: inputs m >= RFIRST/2 are zero at compile time
-            if (m < (PL::RFIRST + 1) / 2 && n < Nu) {
-                if (hasA) va = xa[n];
-                if (hasB) vb = xb[n];
+            if (m < MH) {
+                va = hasA && n < Nu ? la[m] : (R)0;
+                vb = hasB && n < Nu ? lb[m] : (R)0;
             }
             reg[b * PL::RFIRST + m] = {va, vb};
         }
