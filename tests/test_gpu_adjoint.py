@@ -176,6 +176,23 @@ def test_tap_gradient_kernel_fp32_and_fp64(torch):
         assert O.rel_rmse(d32, ref) <= 1e-5, Nu
 
 
+def test_tap_gradient_odd_row_counts(torch):
+    # an odd number of detector rows leaves the last row pair without its
+    # second row (read as row 0 and masked to zero); one row is the extreme
+    for Nu, Nv, Nang in ((37, 3, 3), (64, 1, 1), (100, 1, 5)):
+        g = scan(Nu, Nv, Nang, 8, 8, 8, D=2500.0)
+        st = stages(g)
+        rng = np.random.default_rng(Nu + Nv)
+        q = rng.standard_normal(st.sino_shape())
+        G = rng.standard_normal(st.sino_shape())
+        ref = O.tap_gradient(ogeom(g), q, G)
+        d64 = st.tap_gradient(torch.from_numpy(q).cuda(), torch.from_numpy(G).cuda())
+        d32 = st.tap_gradient(torch.from_numpy(q.astype(np.float32)).cuda(),
+                              torch.from_numpy(G.astype(np.float32)).cuda())
+        assert O.rel_rmse(d64, ref) <= 1e-12, (Nu, Nv, Nang)
+        assert O.rel_rmse(d32, ref) <= 1e-5, (Nu, Nv, Nang)
+
+
 def test_chunked_recomputation_matches_full(torch):
     """cfg4 geometry reduced to 64^3 / 64^2 / 60 projections, chunks of 7."""
     from fdk_autograd import fdk, ramlak_taps
