@@ -896,7 +896,7 @@ def main():
                 raise RuntimeError(f"differentiation leg {name} failed ({r.returncode})")
             out["differentiation"][name] = json.loads(r.stdout.strip().splitlines()[-1])
     if not args.no_toeplitz:
-        # the north star's MFMA alternative to the FFT row filter, measured (torch
+        # the matrix-core (Toeplitz GEMM) alternative to the FFT row filter, measured (torch
         # in a child process, as above)
         r = subprocess.run([sys.executable, os.path.abspath(__file__), "--toeplitz-leg"],
                            capture_output=True, text=True, timeout=300)
