@@ -229,6 +229,9 @@ __global__ void __launch_bounds__(256) fp_lds_kernel(FPArgs<float> a)
         const float w1 = W * fu, w0 = W - W * fu;
         // box byte column (LDS splats) or detector column (HBM splats)
         const int cb = !act ? -1 : bw > 0 ? (iu - bu0) * 4 : iu;
+        // lane t of wave w holds column t here and splats columns 64w..64w+63
+        // below: whether all of them are active is known to the wave
+        const bool wave_active = __all(act);
         __syncthreads();   // every wave is done with the previous angle's records
         // LDS splats take the weights in units of 1/fxs, HBM splats as they are
         const float ws = bw > 0 ? fxs : 1.f;
@@ -249,24 +252,45 @@ __global__ void __launch_bounds__(256) fp_lds_kernel(FPArgs<float> a)
             // per-column mask (64 masks spilled): 44 -> 37 ms for the adjoint of
             // a 256^3 crop of 1024^3 over 1200 projections of 1536^2.
             float4 rn = colrec[64 * wave];
+            if (inner && wave_active) {
+                // every column of the wave active and every slice's taps on
+                // the detector (the box covers the whole z-run, padding slices
+                // included): no lane tests, so no branch splits the waits
+                // (33.8 against 36.7 ms for the crop adjoint above)
 #pragma unroll
-            for (int j = 0; j < 64; ++j) {
-                const float4 r = rn;
-                if (j + 1 < 64) rn = colrec[64 * wave + j + 1];
-                const int cbj = __float_as_int(r.y);
-                const float v = fmaf(azl, r.x, a.cZ);
-                const float g = val[j];
-                // cbj < 0: column inactive at this angle
-                if (cbj < 0 || (!inner && !(v >= 0.f && v <= a.vmax))) continue;
-                const float fv = __builtin_amdgcn_fractf(v);
-                int *p0 = (int *)(lds + (int)v * (4 * FP_P) + rowb + cbj);
-                const float x1 = g * fv, x0 = g - x1;
-                // rounded to nearest: zero-mean errors, which partly cancel
-                // in the sums (truncation biased every sample toward zero)
-                atomicAdd(p0, __float2int_rn(r.z * x0));
-                atomicAdd(p0 + 1, __float2int_rn(r.w * x0));
-                atomicAdd(p0 + FP_P, __float2int_rn(r.z * x1));
-                atomicAdd(p0 + FP_P + 1, __float2int_rn(r.w * x1));
+                for (int j = 0; j < 64; ++j) {
+                    const float4 r = rn;
+                    if (j + 1 < 64) rn = colrec[64 * wave + j + 1];
+                    const float v = fmaf(azl, r.x, a.cZ);
+                    const float g = val[j];
+                    const float fv = __builtin_amdgcn_fractf(v);
+                    int *p0 = (int *)(lds + (int)v * (4 * FP_P) + rowb + __float_as_int(r.y));
+                    const float x1 = g * fv, x0 = g - x1;
+                    atomicAdd(p0, __float2int_rn(r.z * x0));
+                    atomicAdd(p0 + 1, __float2int_rn(r.w * x0));
+                    atomicAdd(p0 + FP_P, __float2int_rn(r.z * x1));
+                    atomicAdd(p0 + FP_P + 1, __float2int_rn(r.w * x1));
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < 64; ++j) {
+                    const float4 r = rn;
+                    if (j + 1 < 64) rn = colrec[64 * wave + j + 1];
+                    const int cbj = __float_as_int(r.y);
+                    const float v = fmaf(azl, r.x, a.cZ);
+                    const float g = val[j];
+                    // cbj < 0: column inactive at this angle
+                    if (cbj < 0 || (!inner && !(v >= 0.f && v <= a.vmax))) continue;
+                    const float fv = __builtin_amdgcn_fractf(v);
+                    int *p0 = (int *)(lds + (int)v * (4 * FP_P) + rowb + cbj);
+                    const float x1 = g * fv, x0 = g - x1;
+                    // rounded to nearest: zero-mean errors, which partly cancel
+                    // in the sums (truncation biased every sample toward zero)
+                    atomicAdd(p0, __float2int_rn(r.z * x0));
+                    atomicAdd(p0 + 1, __float2int_rn(r.w * x0));
+                    atomicAdd(p0 + FP_P, __float2int_rn(r.z * x1));
+                    atomicAdd(p0 + FP_P + 1, __float2int_rn(r.w * x1));
+                }
             }
             __syncthreads();
             // flush: one global atomic per touched sample, rows of 64 lanes
